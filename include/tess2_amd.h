@@ -1,0 +1,187 @@
+/*
+ * tess2_amd.h -- C ABI of the MI355X (gfx950) implementation of tess2's
+ * post-tessellation path: tet circumcenters, Voronoi cell volumes, and the
+ * Voronoi density estimator with its deposit onto a regular float32 grid.
+ *
+ * Plain C types only (pointers + sizes); every entry point returns 0 on
+ * success and a negative TESS2_E* code on failure (never aborts), with a
+ * message available from tess2_last_error().  Host pointers are the
+ * caller's; the library owns its device buffers and its HIP stream.
+ *
+ * Reference interfaces replaced (paths relative to diatomic/tess2):
+ *   tess2_fill_circumcenters  <- fill_circumcenters()  src/volume.cpp:6-11,
+ *                                include/tess/volume.h:7-10
+ *   tess2_volumes             <- volume() over all original cells,
+ *                                src/volume.cpp:13-54, include/tess/volume.h:12-16
+ *   tess2_dense               <- dense() + est_dense() + recvd_pts(),
+ *                                src/dense.cpp:30-202, include/tess/dense.hpp:75-98
+ *   tess2_grid_step_params    <- GridStepParams()  src/dense.cpp:1712-1767
+ *   tess2_block_grid_params   <- BlockGridParams() src/dense.cpp:575-648
+ *   tess2_write_grid          <- WriteGrid() raw float32 output,
+ *                                src/dense.cpp:751-870
+ * The staged (device-resident) calls have no reference counterpart: they
+ * split dense() into upload / run / download so that a caller can keep the
+ * tessellation resident in HBM across runs.
+ */
+#ifndef TESS2_AMD_H
+#define TESS2_AMD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* tet_t, include/tess/tet.h:4-7: tets[i] lies opposite verts[i], -1 = hull */
+typedef struct tess2_tet_t {
+    int verts[4];
+    int tets[4];
+} tess2_tet_t;
+
+/* The dblock_t fields the path reads (include/tess/delaunay.h:38-63) plus
+ * DBlock::bounds (include/tess/delaunay.hpp:18-23).  Original particles come
+ * first, then ghosts. */
+typedef struct tess2_block_t {
+    int gid;
+    int num_orig_particles;
+    int num_particles;
+    const float *particles;          /* 3*num_particles floats, xyz */
+    int num_tets;
+    const tess2_tet_t *tets;
+    const int *vert_to_tet;          /* num_particles entries, -1 = none */
+    float bounds_min[3];
+    float bounds_max[3];
+} tess2_block_t;
+
+/* estimator algorithm, enum alg in include/tess/dense.hpp:33-38 */
+enum { TESS2_DENSE_TESS = 0, TESS2_DENSE_CIC = 1 };
+
+/* grid-point assignment of the TESS estimator */
+enum {
+    TESS2_ASSIGN_REFERENCE_SCAN = 0, /* the reference scanline search, bit-exact */
+    TESS2_ASSIGN_EXACT_VORONOI = 1   /* nearest site, ties to the lowest id    */
+};
+
+/* arithmetic of the EXACT_VORONOI assignment */
+enum {
+    TESS2_PREC_MIXED = 0,  /* FP32 filters, FP64 fallback near ties */
+    TESS2_PREC_FP64 = 1    /* every predicate in FP64               */
+};
+
+/* per-cell outcome codes written by tess2_dense_cells() */
+enum {
+    TESS2_CELL_SCAN = 0,
+    TESS2_CELL_CIC = 1,
+    TESS2_CELL_INCOMPLETE = 2,
+    TESS2_CELL_OUTSIDE = 3,
+    TESS2_CELL_DEGENERATE = 4
+};
+
+/* error codes */
+enum {
+    TESS2_OK = 0,
+    TESS2_E_ARG = -1,
+    TESS2_E_HIP = -2,
+    TESS2_E_CAPACITY = -3,   /* a per-cell capacity was exceeded */
+    TESS2_E_NOMEM = -4,
+    TESS2_E_IO = -5
+};
+
+/* counters of one dense run (the reference prints tot_mass / check_mass /
+ * max_dense, src/dense.cpp:25-27,1284-1333) */
+typedef struct tess2_stats_t {
+    long long cells_scan;
+    long long cells_cic;
+    long long cells_incomplete;
+    long long cells_outside;
+    long long cells_degenerate;
+    long long points_deposited;      /* in-block deposits                     */
+    long long points_offblock;       /* points outside the block's bounds     */
+    long long points_offgrid;        /* in bounds but outside the block grid  */
+    long long points_shared;         /* deposits folded by the ordered pass   */
+    long long fp64_fallbacks;        /* EXACT_VORONOI mixed: FP64 re-checks   */
+    double tot_mass;                 /* sum of deposited point masses         */
+    float max_dense;
+    float device_ms;                 /* device time of the last run          */
+} tess2_stats_t;
+
+typedef struct tess2_ctx tess2_ctx;
+
+int tess2_create(tess2_ctx **ctx, int device);
+int tess2_destroy(tess2_ctx *ctx);
+const char *tess2_last_error(const tess2_ctx *ctx);
+
+/* fill_circumcenters(): one circumcenter per tet, float32, bit-exact with
+ * src/tet.cpp:37-66.  out holds 3*num_tets floats. */
+int tess2_fill_circumcenters(tess2_ctx *ctx, const tess2_tet_t *tets, int num_tets,
+                             const float *particles, int num_particles, float *out);
+
+/* volume() for every original cell v < num_orig_particles; -1 for an
+ * infinite cell (and for a particle in no tet). */
+int tess2_volumes(tess2_ctx *ctx, const tess2_block_t *block, float *out);
+
+/* GridStepParams() / BlockGridParams() host helpers (same arguments). */
+void tess2_grid_step_params(int num_given_bounds, const float *given_mins,
+                            const float *given_maxs, const float *data_mins,
+                            const float *data_maxs, float *grid_phys_mins,
+                            float *grid_phys_maxs, float *grid_step_size,
+                            const int *glo_num_idx);
+void tess2_block_grid_params(const float *bounds_min, const float *bounds_max,
+                             int *block_min_idx, int *block_max_idx,
+                             int *block_num_idx, const float *grid_phys_mins,
+                             const float *grid_step_size, float eps,
+                             const float *data_mins, const float *data_maxs,
+                             const int *glo_num_idx);
+
+/* dense(): density of one or more blocks on their block grids.
+ * Outputs data_mins/maxs, grid_phys_mins/maxs, grid_step_size as the
+ * reference does; density_out[b] must hold the block's grid
+ * (block_num_idx product, or x*y when projecting).  assignment /
+ * precision select the TESS estimator variant (REFERENCE_SCAN is the
+ * reference's algorithm).  stats may be NULL. */
+int tess2_dense(tess2_ctx *ctx, int alg, int num_given_bounds,
+                const float *given_mins, const float *given_maxs, int project,
+                const float *proj_plane, float mass, float *data_mins,
+                float *data_maxs, float *grid_phys_mins, float *grid_phys_maxs,
+                float *grid_step_size, float eps, const int *glo_num_idx,
+                const tess2_block_t *blocks, int nblocks, int assignment,
+                int precision, float **density_out, tess2_stats_t *stats);
+
+/* Per-cell detail of the last tess2_dense / tess2_run call for block b:
+ * status (TESS2_CELL_*) and point count per original cell. */
+int tess2_dense_cells(tess2_ctx *ctx, int b, int *status, int *npts);
+
+/* WriteGrid(): raw float32, native endian, C order (x fastest), no header;
+ * the file is truncated first (src/dense.cpp:781-870).  With project, the
+ * 2-D [y][x] plane. */
+int tess2_write_grid(const char *path, const float *density, const int *glo_num_idx,
+                     int project);
+
+/* ---- staged (device-resident) interface ---- */
+
+/* Upload blocks once; later runs read them from HBM. */
+int tess2_stage(tess2_ctx *ctx, const tess2_block_t *blocks, int nblocks);
+
+/* Run the whole path on the staged blocks, asynchronously on the context
+ * stream: circumcenters -> stars -> volumes -> face planes -> assignment ->
+ * deposit.  Arguments as tess2_dense (data bounds are taken from the staged
+ * blocks).  Results stay on the device until tess2_fetch. */
+int tess2_run(tess2_ctx *ctx, int alg, int num_given_bounds, const float *given_mins,
+              const float *given_maxs, int project, const float *proj_plane,
+              float mass, float eps, const int *glo_num_idx, int assignment,
+              int precision, int want_volumes);
+
+/* Wait for the last run; fill stats (device_ms = event time of the run). */
+int tess2_sync(tess2_ctx *ctx, tess2_stats_t *stats);
+
+/* Copy results of block b to the host (NULL pointers are skipped). */
+int tess2_fetch(tess2_ctx *ctx, int b, float *density, float *volumes,
+                float *circumcenters);
+
+/* Average device time per launch (ms) of the dominant kernel over the
+ * runs since the last call, measured with HIP events on the library
+ * stream, and the number of launches averaged. */
+int tess2_kernel_time(tess2_ctx *ctx, float *avg_ms, int *launches);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,306 @@
+"""ctypes wrapper of the CPU oracle (liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: the checker for the HIP library.  Only tests/,
+the smoke check and the CPU-baseline leg of bench.py import this module.
+Each wrapper names the reference function it restates; the C source cites
+the reference lines.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+F32P = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
+F64P = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+I32P = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+
+CELL_SCAN, CELL_CIC, CELL_INCOMPLETE, CELL_OUTSIDE, CELL_DEGENERATE = range(5)
+
+
+def build():
+    subprocess.check_call(["make", "-s", "-C", _HERE])
+
+
+class OBlock(C.Structure):
+    _fields_ = [("num_orig_particles", C.c_int), ("num_particles", C.c_int),
+                ("particles", C.c_void_p), ("num_tets", C.c_int),
+                ("tets", C.c_void_p), ("vert_to_tet", C.c_void_p),
+                ("bounds_min", C.c_float * 3), ("bounds_max", C.c_float * 3)]
+
+
+class OArgs(C.Structure):
+    _fields_ = [("project", C.c_int), ("proj_plane", C.c_float * 3),
+                ("mass", C.c_float), ("data_mins", C.c_float * 3),
+                ("data_maxs", C.c_float * 3), ("grid_phys_mins", C.c_float * 3),
+                ("grid_step_size", C.c_float * 3), ("eps", C.c_float),
+                ("glo_num_idx", C.c_int * 3)]
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB):
+            build()
+        L = C.CDLL(_LIB)
+        vp = C.c_void_p
+        L.o_circumcenter.argtypes = [F32P, I32P, F32P]
+        L.o_fill_circumcenters.argtypes = [I32P, C.c_int, F32P, F32P]
+        L.o_neighbor_edges.argtypes = [C.c_int, I32P, C.c_int, I32P, I32P, C.c_int,
+                                       C.POINTER(C.c_int)]
+        L.o_complete.argtypes = [C.c_int, I32P, C.c_int, C.c_int]
+        L.o_fill_edge_link.argtypes = [I32P, C.c_int, C.c_int, C.c_int, C.c_int, I32P]
+        L.o_volume.restype = C.c_float
+        L.o_volume.argtypes = [C.c_int, I32P, I32P, C.c_int, F32P, F32P]
+        L.o_volumes.argtypes = [C.POINTER(OBlock), F32P, F32P]
+        L.o_newell_normal.argtypes = [F32P, C.c_int, F32P]
+        L.o_pt_in_cell.argtypes = [F32P, F32P, F32P, C.c_int, C.c_float]
+        L.o_idx2phys.argtypes = [I32P, F32P, F32P, F32P]
+        L.o_phys2idx.argtypes = [F32P, I32P, F32P, F32P]
+        L.o_index.argtypes = [I32P, I32P, C.c_int, F32P]
+        L.o_grid_step_params.argtypes = [C.c_int, F32P, F32P, F32P, F32P, F32P, F32P,
+                                         F32P, I32P]
+        L.o_block_grid_params.argtypes = [F32P, F32P, I32P, I32P, I32P, F32P, F32P,
+                                          C.c_float, F32P, F32P, I32P]
+        L.o_distribute_cic.argtypes = [F32P, C.c_float, I32P, F32P, F32P, F32P,
+                                       C.c_float]
+        L.o_cell_bounds.argtypes = [C.POINTER(OBlock), C.c_int, F32P, F32P, F32P, F32P,
+                                    C.c_int]
+        L.o_cell_points.argtypes = [C.POINTER(OBlock), C.POINTER(OArgs), C.c_int, I32P,
+                                    F64P, C.c_int, C.POINTER(C.c_int)]
+        L.o_iterate_cells.restype = C.c_long
+        L.o_iterate_cells.argtypes = [C.POINTER(OBlock), C.POINTER(OArgs), I32P, I32P,
+                                      F32P, vp, vp, vp, vp, vp, C.c_long, vp]
+        L.o_iterate_cells_cic.restype = C.c_long
+        L.o_iterate_cells_cic.argtypes = [C.POINTER(OBlock), C.POINTER(OArgs), I32P, I32P,
+                                          F32P, vp, vp, vp, C.c_long, vp]
+        L.o_recvd_pts.argtypes = [I32P, F64P, C.c_long, I32P, I32P, C.POINTER(OArgs), F32P]
+        L.o_exact_owner.argtypes = [F32P, C.c_int, I32P, F32P, F32P, I32P]
+        _lib = L
+    return _lib
+
+
+def _f3(x):
+    return np.ascontiguousarray(x, dtype=np.float32).reshape(3)
+
+
+def _i3(x):
+    return np.ascontiguousarray(x, dtype=np.int32).reshape(3)
+
+
+class Block:
+    """Holds the arrays of one block alive and exposes the C struct."""
+
+    def __init__(self, particles, tets, vert_to_tet, num_orig, bounds_min, bounds_max):
+        self.particles = np.ascontiguousarray(particles, dtype=np.float32).reshape(-1)
+        self.tets = np.ascontiguousarray(tets, dtype=np.int32).reshape(-1, 8)
+        self.vert_to_tet = np.ascontiguousarray(vert_to_tet, dtype=np.int32)
+        self.num_orig = int(num_orig)
+        self.cs = OBlock()
+        self.cs.num_orig_particles = self.num_orig
+        self.cs.num_particles = self.particles.size // 3
+        self.cs.particles = self.particles.ctypes.data
+        self.cs.num_tets = self.tets.shape[0]
+        self.cs.tets = self.tets.ctypes.data
+        self.cs.vert_to_tet = self.vert_to_tet.ctypes.data
+        self.cs.bounds_min[:] = [float(x) for x in _f3(bounds_min)]
+        self.cs.bounds_max[:] = [float(x) for x in _f3(bounds_max)]
+
+    @classmethod
+    def from_host(cls, hb):
+        """From a tess2_amd.ingest.HostBlock."""
+        return cls(hb.particles, hb.tets, hb.vert_to_tet, hb.num_orig,
+                   hb.bounds_min, hb.bounds_max)
+
+
+def make_args(mass, data_mins, data_maxs, gmin, step, eps, gnum, project=False,
+              proj_plane=(0.0, 0.0, 1.0)):
+    a = OArgs()
+    a.project = int(bool(project))
+    a.proj_plane[:] = [float(x) for x in _f3(proj_plane)]
+    a.mass = float(np.float32(mass))
+    a.data_mins[:] = [float(x) for x in _f3(data_mins)]
+    a.data_maxs[:] = [float(x) for x in _f3(data_maxs)]
+    a.grid_phys_mins[:] = [float(x) for x in _f3(gmin)]
+    a.grid_step_size[:] = [float(x) for x in _f3(step)]
+    a.eps = float(np.float32(eps))
+    a.glo_num_idx[:] = [int(x) for x in _i3(gnum)]
+    return a
+
+
+# ---- tet geometry ---------------------------------------------------------
+
+def circumcenters(tets, particles):
+    """fill_circumcenters(), src/volume.cpp:6-11."""
+    tets = np.ascontiguousarray(tets, dtype=np.int32).reshape(-1, 8)
+    p = np.ascontiguousarray(particles, dtype=np.float32).reshape(-1)
+    out = np.zeros(tets.shape[0] * 3, np.float32)
+    lib().o_fill_circumcenters(tets.reshape(-1), tets.shape[0], p, out)
+    return out.reshape(-1, 3)
+
+
+def neighbor_edges(v, tets, t, cap=4096):
+    """neighbor_edges(), src/tet.cpp:228-270 -> (finite, [(u, ut), ...])."""
+    tets = np.ascontiguousarray(tets, dtype=np.int32).reshape(-1)
+    u = np.zeros(cap, np.int32)
+    ut = np.zeros(cap, np.int32)
+    n = C.c_int()
+    fin = lib().o_neighbor_edges(int(v), tets, int(t), u, ut, cap, C.byref(n))
+    return fin, list(zip(u[:n.value].tolist(), ut[:n.value].tolist()))
+
+
+def complete(v, tets, t):
+    tets = np.ascontiguousarray(tets, dtype=np.int32).reshape(-1)
+    return lib().o_complete(int(v), tets, tets.size // 8, int(t))
+
+
+def edge_link(v, u, ut, tets, cap=4096):
+    """fill_edge_link(), src/tet.cpp:389-409."""
+    tets = np.ascontiguousarray(tets, dtype=np.int32).reshape(-1)
+    out = np.zeros(cap, np.int32)
+    n = lib().o_fill_edge_link(out, cap, int(v), int(u), int(ut), tets)
+    return out[:n].tolist()
+
+
+def volumes(block, cc=None):
+    """volume() for every original cell, src/volume.cpp:13-54."""
+    if cc is None:
+        cc = circumcenters(block.tets, block.particles)
+    cc = np.ascontiguousarray(cc, dtype=np.float32).reshape(-1)
+    out = np.zeros(block.num_orig, np.float32)
+    lib().o_volumes(C.byref(block.cs), cc, out)
+    return out
+
+
+# ---- density helpers -------------------------------------------------------
+
+def newell_normal(verts):
+    v = np.ascontiguousarray(verts, dtype=np.float32).reshape(-1)
+    out = np.zeros(3, np.float32)
+    lib().o_newell_normal(v, v.size // 3, out)
+    return out
+
+
+def pt_in_cell(pt, normals, f0, eps):
+    n = np.ascontiguousarray(normals, dtype=np.float32).reshape(-1)
+    f = np.ascontiguousarray(f0, dtype=np.float32).reshape(-1)
+    return lib().o_pt_in_cell(_f3(pt), n, f, n.size // 3, float(np.float32(eps)))
+
+
+def grid_step_params(data_mins, data_maxs, gnum, given_mins=None, given_maxs=None,
+                     num_given=0):
+    gmn = np.zeros(3, np.float32)
+    gmx = np.zeros(3, np.float32)
+    st = np.zeros(3, np.float32)
+    gi = _f3(given_mins if given_mins is not None else (0, 0, 0))
+    ga = _f3(given_maxs if given_maxs is not None else (0, 0, 0))
+    lib().o_grid_step_params(int(num_given), gi, ga, _f3(data_mins), _f3(data_maxs),
+                             gmn, gmx, st, _i3(gnum))
+    return gmn, gmx, st
+
+
+def block_grid_params(bmin, bmax, gmin, step, eps, data_mins, data_maxs, gnum):
+    mn = np.zeros(3, np.int32)
+    mx = np.zeros(3, np.int32)
+    nm = np.zeros(3, np.int32)
+    lib().o_block_grid_params(_f3(bmin), _f3(bmax), mn, mx, nm, _f3(gmin), _f3(step),
+                              float(np.float32(eps)), _f3(data_mins), _f3(data_maxs),
+                              _i3(gnum))
+    return mn, mx, nm
+
+
+def distribute_cic(pt, scalar, step, gmin, eps):
+    idx = np.zeros(24, np.int32)
+    m = np.zeros(8, np.float32)
+    lib().o_distribute_cic(_f3(pt), float(np.float32(scalar)), idx, m, _f3(step),
+                           _f3(gmin), float(np.float32(eps)))
+    return idx.reshape(8, 3), m
+
+
+def cell_bounds(block, cell, cap=1024):
+    bmin = np.zeros(3, np.float32)
+    bmax = np.zeros(3, np.float32)
+    nrm = np.zeros(3 * cap, np.float32)
+    f0 = np.zeros(3 * cap, np.float32)
+    nf = lib().o_cell_bounds(C.byref(block.cs), int(cell), bmin, bmax, nrm, f0, cap)
+    return bmin, bmax, nrm[:3 * nf].reshape(-1, 3), f0[:3 * nf].reshape(-1, 3)
+
+
+def cell_points(block, args, cell, cap=1 << 16):
+    idx = np.zeros(3 * cap, np.int32)
+    m = np.zeros(cap, np.float64)
+    st = C.c_int()
+    n = lib().o_cell_points(C.byref(block.cs), C.byref(args), int(cell), idx, m, cap,
+                            C.byref(st))
+    if n < 0:
+        raise RuntimeError("cell_points capacity")
+    return st.value, idx[:3 * n].reshape(-1, 3), m[:n]
+
+
+def iterate_cells(block, args, bmin_idx, bnum_idx, density=None, off_cap=1 << 20,
+                  cic=False):
+    """IterateCells (src/dense.cpp:220-313) or IterateCellsCic (:486-562).
+
+    Returns (density, status, npts, offblock(idx3, mass, cell), tot_mass)."""
+    bnum = _i3(bnum_idx)
+    npts_grid = int(bnum[0]) * int(bnum[1]) * (1 if args.project else int(bnum[2]))
+    if density is None:
+        density = np.zeros(npts_grid, np.float32)
+    nc = block.num_orig
+    status = np.full(nc, -1, np.int32)
+    npts = np.zeros(nc, np.int32)
+    oidx = np.zeros(3 * off_cap, np.int32)
+    om = np.zeros(off_cap, np.float64)
+    oc = np.zeros(off_cap, np.int32)
+    tm = C.c_double(0.0)
+    L = lib()
+    if cic:
+        noff = L.o_iterate_cells_cic(C.byref(block.cs), C.byref(args), _i3(bmin_idx), bnum,
+                                     density, oidx.ctypes.data, om.ctypes.data,
+                                     oc.ctypes.data, off_cap, C.addressof(tm))
+    else:
+        noff = L.o_iterate_cells(C.byref(block.cs), C.byref(args), _i3(bmin_idx), bnum,
+                                 density, status.ctypes.data, npts.ctypes.data,
+                                 oidx.ctypes.data, om.ctypes.data, oc.ctypes.data,
+                                 off_cap, C.addressof(tm))
+    k = min(noff, off_cap)
+    off = (oidx[:3 * k].reshape(-1, 3), om[:k], oc[:k])
+    return density, status, npts, off, tm.value
+
+
+def recvd_pts(idx3, mass, bmin_idx, bnum_idx, args, density):
+    idx3 = np.ascontiguousarray(idx3, dtype=np.int32).reshape(-1)
+    mass = np.ascontiguousarray(mass, dtype=np.float64)
+    lib().o_recvd_pts(idx3, mass, mass.size, _i3(bmin_idx), _i3(bnum_idx),
+                      C.byref(args), density)
+
+
+def exact_owner(particles, gnum, step, gmin):
+    """Nearest site of every grid point (ties: lowest id), FP64 distances."""
+    p = np.ascontiguousarray(particles, dtype=np.float32).reshape(-1)
+    g = _i3(gnum)
+    out = np.zeros(int(g[0]) * int(g[1]) * int(g[2]), np.int32)
+    lib().o_exact_owner(p, p.size // 3, g, _f3(step), _f3(gmin), out)
+    return out
+
+
+def dense_single(block, gnum, mass=1.0, eps=1e-4, cic=False, project=False,
+                 proj_plane=(0.0, 0.0, 1.0), num_given=0, given_mins=None,
+                 given_maxs=None):
+    """dense() for one block whose bounds are the data bounds (DataBounds over a
+    single block, src/dense.cpp:1221-1275), no neighbours: off-block points are
+    returned, not deposited.  Returns a dict."""
+    dmin = np.array(block.cs.bounds_min[:], np.float32)
+    dmax = np.array(block.cs.bounds_max[:], np.float32)
+    gmin, gmax, step = grid_step_params(dmin, dmax, gnum, given_mins, given_maxs,
+                                        num_given)
+    bmn, bmx, bnum = block_grid_params(dmin, dmax, gmin, step, eps, dmin, dmax, gnum)
+    args = make_args(mass, dmin, dmax, gmin, step, eps, gnum, project, proj_plane)
+    den, status, npts, off, tm = iterate_cells(block, args, bmn, bnum, cic=cic)
+    return dict(density=den, status=status, npts=npts, offblock=off, tot_mass=tm,
+                grid_phys_mins=gmin, grid_phys_maxs=gmax, grid_step_size=step,
+                block_min_idx=bmn, block_max_idx=bmx, block_num_idx=bnum, args=args)

@@ -1,0 +1,1005 @@
+/*
+ * tess_oracle.c -- CPU oracle for the tess2 post-tessellation path.
+ *
+ * TEST INFRASTRUCTURE ONLY: the checker for the HIP library, loaded by the
+ * tests, the smoke check and the benchmark's CPU-baseline leg, never by the
+ * product path.
+ *
+ * A clean-room restatement of the reference's algorithm, following
+ * src/tet.cpp, src/volume.cpp and src/dense.cpp operation by operation.  Each
+ * function cites the reference lines it follows.  Float arithmetic is IEEE
+ * binary32 in the reference's evaluation order; where the C++ source mixes
+ * float and double (a double literal, a double-typed struct field, a call to
+ * the double sqrt) the promotion is written out explicitly here.
+ *
+ * Parity status of each part (what pins it) is listed in the test files
+ * tests/test_oracle_*.py next to the checks themselves.
+ */
+#include "tess_oracle.h"
+
+#include <float.h>
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* ------------------------------------------------------------------------ */
+/* small growable int buffer (scratch for the BFS / edge links)              */
+/* ------------------------------------------------------------------------ */
+typedef struct {
+    int *p;
+    int n, cap;
+} ibuf;
+
+static void ib_push(ibuf *b, int x)
+{
+    if (b->n == b->cap) {
+        b->cap = b->cap ? 2 * b->cap : 64;
+        b->p = (int *)realloc(b->p, sizeof(int) * (size_t)b->cap);
+    }
+    b->p[b->n++] = x;
+}
+
+static int ib_has(const ibuf *b, int x)
+{
+    for (int i = 0; i < b->n; ++i)
+        if (b->p[i] == x)
+            return 1;
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* tet geometry                                                             */
+/* ------------------------------------------------------------------------ */
+
+/* norm(): |x|^2, src/tet.cpp:122-128 (res starts at 0 and accumulates) */
+static float o_norm(const float *x)
+{
+    float r = 0.0f;
+    for (int i = 0; i < 3; ++i)
+        r += x[i] * x[i];
+    return r;
+}
+
+/* cross(): src/tet.cpp:131-136 */
+static void o_cross(float *r, const float *u, const float *v)
+{
+    r[0] = u[1] * v[2] - u[2] * v[1];
+    r[1] = u[2] * v[0] - u[0] * v[2];
+    r[2] = u[0] * v[1] - u[1] * v[0];
+}
+
+/* determinant(): six triple products, left to right, src/tet.cpp:139-143 */
+static float o_det(const float *t, const float *u, const float *v)
+{
+    return t[0] * u[1] * v[2] + u[0] * v[1] * t[2] + v[0] * t[1] * u[2] -
+           v[0] * u[1] * t[2] - u[0] * t[1] * v[2] - t[0] * v[1] * u[2];
+}
+
+/* distance(): sqrt of the float sum; the float result of the double sqrt
+ * equals the correctly rounded float sqrt, src/tet.cpp:146-153 */
+static float o_distance(const float *u, const float *v)
+{
+    float n = 0.0f;
+    for (int i = 0; i < 3; ++i)
+        n += (u[i] - v[i]) * (u[i] - v[i]);
+    return sqrtf(n);
+}
+
+/* circumcenter(): src/tet.cpp:37-66 */
+void o_circumcenter(float *center, const o_tet_t *tet, const float *particles)
+{
+    const float *a = &particles[3 * tet->verts[0]];
+    const float *b = &particles[3 * tet->verts[1]];
+    const float *c = &particles[3 * tet->verts[2]];
+    const float *d = &particles[3 * tet->verts[3]];
+    float t[3], u[3], v[3];
+    for (int i = 0; i < 3; ++i) {
+        t[i] = a[i] - d[i];
+        u[i] = b[i] - d[i];
+        v[i] = c[i] - d[i];
+    }
+    float nt = o_norm(t), nu = o_norm(u), nv = o_norm(v);
+    float den = 2.0f * o_det(t, u, v);
+    float uv[3], vt[3], tu[3];
+    o_cross(uv, u, v);
+    o_cross(vt, v, t);
+    o_cross(tu, t, u);
+    for (int i = 0; i < 3; ++i)
+        center[i] = d[i] + (nt * uv[i] + nu * vt[i] + nv * tu[i]) / den;
+}
+
+/* fill_circumcenters(): src/volume.cpp:6-11 */
+void o_fill_circumcenters(const o_tet_t *tets, int num_tets,
+                          const float *particles, float *out)
+{
+    for (int i = 0; i < num_tets; ++i)
+        o_circumcenter(&out[3 * i], &tets[i], particles);
+}
+
+/* BFS over the star of v from tet t, src/tet.cpp:228-270.  The queue may
+ * hold a tet several times; a tet is expanded on its first pop.  Fills
+ * (u, ut) pairs in first-seen order.  Returns finite (1/0). */
+static int star_bfs(int v, const o_tet_t *tets, int t, ibuf *q, ibuf *vis,
+                    ibuf *vverts, ibuf *nu, ibuf *nt)
+{
+    int finite = 1;
+    q->n = vis->n = vverts->n = 0;
+    if (nu) nu->n = 0;
+    if (nt) nt->n = 0;
+    ib_push(q, t);
+    for (int head = 0; head < q->n; ++head) {
+        int cur = q->p[head];
+        if (ib_has(vis, cur))
+            continue;
+        ib_push(vis, cur);
+        for (int i = 0; i < 4; ++i) {
+            int u = tets[cur].verts[i];
+            if (u != v) {
+                if (!ib_has(vverts, u)) {
+                    ib_push(vverts, u);
+                    if (nu) ib_push(nu, u);
+                    if (nt) ib_push(nt, cur);
+                }
+                int next = tets[cur].tets[i];
+                if (next == -1)
+                    finite = 0;
+                else
+                    ib_push(q, next);
+            }
+        }
+    }
+    return finite;
+}
+
+/* scratch buffers shared by the per-cell routines (single-threaded oracle) */
+static ibuf g_q, g_vis, g_vv, g_nu, g_nt, g_link;
+
+int o_neighbor_edges(int v, const o_tet_t *tets, int t, int *nbr_u, int *nbr_t,
+                     int cap, int *num_nbrs)
+{
+    int finite = star_bfs(v, tets, t, &g_q, &g_vis, &g_vv, &g_nu, &g_nt);
+    *num_nbrs = g_nu.n;
+    if (g_nu.n > cap)
+        return -1;
+    for (int i = 0; i < g_nu.n; ++i) {
+        nbr_u[i] = g_nu.p[i];
+        nbr_t[i] = g_nt.p[i];
+    }
+    return finite;
+}
+
+/* complete(): same BFS, early exit on a hull neighbour, src/tet.cpp:337-378 */
+int o_complete(int v, const o_tet_t *tets, int num_tets, int t)
+{
+    (void)num_tets;
+    g_q.n = g_vis.n = 0;
+    ib_push(&g_q, t);
+    for (int head = 0; head < g_q.n; ++head) {
+        int cur = g_q.p[head];
+        if (ib_has(&g_vis, cur))
+            continue;
+        ib_push(&g_vis, cur);
+        for (int i = 0; i < 4; ++i) {
+            int u = tets[cur].verts[i];
+            if (u != v) {
+                int next = tets[cur].tets[i];
+                if (next == -1)
+                    return 0;
+                ib_push(&g_q, next);
+            }
+        }
+    }
+    return 1;
+}
+
+/* circulate_start(): first slot holding neither edge vertex,
+ * src/tet.cpp:164-173 */
+static int circ_start(const o_tet_t *tets, int t, int x, int y)
+{
+    for (int i = 0; i < 4; ++i) {
+        int w = tets[t].verts[i];
+        if (w != x && w != y)
+            return i;
+    }
+    return -1;
+}
+
+/* circulate_next(): cross the face opposite slot wi, src/tet.cpp:185-215 */
+static void circ_next(int *next_t, int *next_v, const o_tet_t *tets, int t,
+                      int wi, int x, int y)
+{
+    const o_tet_t *cur = tets + t;
+    int nv = -1;
+    for (int i = 0; i < 4; ++i) {
+        int w = cur->verts[i];
+        if (i != wi && w != x && w != y) {
+            nv = w;
+            break;
+        }
+    }
+    *next_t = cur->tets[wi];
+    *next_v = -1;
+    if (*next_t < 0)
+        return;
+    const o_tet_t *nx = tets + *next_t;
+    for (int i = 0; i < 4; ++i)
+        if (nx->verts[i] == nv) {
+            *next_v = i;
+            return;
+        }
+}
+
+/* fill_edge_link(): ring of tets around edge (v,u) from ut,
+ * src/tet.cpp:389-409 */
+static void edge_link(ibuf *link, int v, int u, int ut, const o_tet_t *tets)
+{
+    link->n = 0;
+    int wi = circ_start(tets, ut, v, u);
+    int t = ut;
+    for (;;) {
+        ib_push(link, t);
+        if (wi < 0)     /* malformed ring (reference: assert); stop here */
+            break;
+        int nt, nwi;
+        circ_next(&nt, &nwi, tets, t, wi, v, u);
+        if (nt == ut || nt == -1)
+            break;
+        t = nt;
+        wi = nwi;
+    }
+}
+
+int o_fill_edge_link(int *link, int cap, int v, int u, int ut,
+                     const o_tet_t *tets)
+{
+    edge_link(&g_link, v, u, ut, tets);
+    if (g_link.n > cap)
+        return -1;
+    memcpy(link, g_link.p, sizeof(int) * (size_t)g_link.n);
+    return g_link.n;
+}
+
+/* ------------------------------------------------------------------------ */
+/* volume(): src/volume.cpp:13-54                                            */
+/* ------------------------------------------------------------------------ */
+static float volume_impl(int v, int vt, const o_tet_t *tets,
+                         const float *particles, const float *cc)
+{
+    int finite = star_bfs(v, tets, vt, &g_q, &g_vis, &g_vv, &g_nu, &g_nt);
+    if (!finite)
+        return -1.0f;
+    float vol = 0.0f;
+    for (int k = 0; k < g_nu.n; ++k) {
+        int u = g_nu.p[k];
+        int ut = g_nt.p[k];
+        edge_link(&g_link, v, u, ut, tets);
+        float area = 0.0f;
+        int a = g_link.p[0];
+        for (int i = 1; i < g_link.n - 1; ++i) {
+            int bb = g_link.p[i];
+            int c = g_link.p[i + 1];
+            float ab[3], ac[3], cp[3];
+            for (int j = 0; j < 3; ++j) {
+                ab[j] = cc[3 * bb + j] - cc[3 * a + j];
+                ac[j] = cc[3 * c + j] - cc[3 * a + j];
+            }
+            o_cross(cp, ab, ac);
+            /* area += sqrt(norm(cp))/2 with the double sqrt: the float sum is
+             * promoted, the double result rounded back (volume.cpp:45) */
+            area = (float)((double)area + sqrt((double)o_norm(cp)) / 2.0);
+        }
+        float dist = o_distance(&particles[3 * u], &particles[3 * v]);
+        vol += area * dist / 6.0f;
+    }
+    return vol;
+}
+
+float o_volume(int v, const int *vert_to_tet, const o_tet_t *tets,
+               int num_tets, const float *particles, const float *cc)
+{
+    (void)num_tets;
+    int vt = vert_to_tet[v];
+    if (vt < 0)     /* reference: undefined (tets[-1]); defined here as -1 */
+        return -1.0f;
+    return volume_impl(v, vt, tets, particles, cc);
+}
+
+void o_volumes(const o_block_t *b, const float *cc, float *out)
+{
+    for (int v = 0; v < b->num_orig_particles; ++v)
+        out[v] = o_volume(v, b->vert_to_tet, b->tets, b->num_tets,
+                          b->particles, cc);
+}
+
+/* ------------------------------------------------------------------------ */
+/* density helpers                                                          */
+/* ------------------------------------------------------------------------ */
+
+/* NewellNormal(): src/dense.cpp:1131-1162 */
+void o_newell_normal(const float *verts, int num_verts, float *normal)
+{
+    normal[0] = 0.0f;
+    normal[1] = 0.0f;
+    normal[2] = 0.0f;
+    for (int i = 0; i < num_verts; i++) {
+        int cur = i;
+        int next = (i + 1) % num_verts;
+        normal[0] += (verts[3 * cur + 1] - verts[3 * next + 1]) *
+                     (verts[3 * cur + 2] + verts[3 * next + 2]);
+        normal[1] += (verts[3 * cur + 2] - verts[3 * next + 2]) *
+                     (verts[3 * cur + 0] + verts[3 * next + 0]);
+        normal[2] += (verts[3 * cur + 0] - verts[3 * next + 0]) *
+                     (verts[3 * cur + 1] + verts[3 * next + 1]);
+    }
+    float mag = sqrtf(normal[0] * normal[0] + normal[1] * normal[1] +
+                      normal[2] * normal[2]);
+    normal[0] /= mag;
+    normal[1] /= mag;
+    normal[2] /= mag;
+    normal[0] = -normal[0];
+    normal[1] = -normal[1];
+    normal[2] = -normal[2];
+}
+
+/* PtInCell(): all faces whose |distance| exceeds eps must agree in sign,
+ * src/dense.cpp:1172-1203.  Only the first vertex of each face is read. */
+int o_pt_in_cell(const float *pt, const float *normals, const float *f0,
+                 int num_faces, float eps)
+{
+    int old_sign = 0;
+    for (int k = 0; k < num_faces; k++) {
+        const float *n = &normals[3 * k];
+        const float *f = &f0[3 * k];
+        float dist = n[0] * (pt[0] - f[0]) + n[1] * (pt[1] - f[1]) +
+                     n[2] * (pt[2] - f[2]);
+        if (fabsf(dist) > eps) {
+            int sign = ((double)dist >= 0.0 ? 1 : -1);
+            if (old_sign == 0)
+                old_sign = sign;
+            if (old_sign != sign)
+                return 0;
+        }
+    }
+    return 1;
+}
+
+/* idx2phys(): src/dense.cpp:1098-1106 */
+void o_idx2phys(const int *idx, float *pos, const float *step, const float *gmin)
+{
+    for (int i = 0; i < 3; ++i)
+        pos[i] = (float)idx[i] * step[i] + gmin[i];
+}
+
+/* phys2idx(): truncating conversion, src/dense.cpp:1117-1125 */
+void o_phys2idx(const float *pos, int *idx, const float *step, const float *gmin)
+{
+    for (int i = 0; i < 3; ++i)
+        idx[i] = (int)((pos[i] - gmin[i]) / step[i]);
+}
+
+/* index(): src/dense.cpp:1047-1090 (projection variant for the x-y plane) */
+int o_index(const int *bidx, const int *bnum, int project,
+            const float *proj_plane)
+{
+    int idx[3];
+    if (project) {
+        float pl = (float)bidx[0] * proj_plane[0] + (float)bidx[1] * proj_plane[1] +
+                   (float)bidx[2] * proj_plane[2];
+        float sh[3] = {pl * proj_plane[0], pl * proj_plane[1], pl * proj_plane[2]};
+        for (int i = 0; i < 3; ++i)
+            idx[i] = (int)((float)bidx[i] - sh[i]);
+    } else {
+        idx[0] = bidx[0];
+        idx[1] = bidx[1];
+        idx[2] = bidx[2];
+    }
+    return idx[2] * bnum[1] * bnum[0] + idx[1] * bnum[0] + idx[0];
+}
+
+/* GridStepParams(): src/dense.cpp:1712-1767.  The half-padding is computed
+ * with the double literal 2.0, so the padding step runs in double. */
+void o_grid_step_params(int num_given_bounds, const float *given_mins,
+                        const float *given_maxs, const float *data_mins,
+                        const float *data_maxs, float *grid_phys_mins,
+                        float *grid_phys_maxs, float *grid_step_size,
+                        const int *glo_num_idx)
+{
+    float mds = data_maxs[0] - data_mins[0];
+    if (data_maxs[1] - data_mins[1] > mds)
+        mds = data_maxs[1] - data_mins[1];
+    if (data_maxs[2] - data_mins[2] > mds)
+        mds = data_maxs[2] - data_mins[2];
+    for (int i = 0; i < 3; ++i) {
+        float pad = mds - (data_maxs[i] - data_mins[i]);
+        grid_phys_mins[i] = (float)((double)data_mins[i] - (double)pad / 2.0);
+        grid_phys_maxs[i] = (float)((double)data_maxs[i] + (double)pad / 2.0);
+    }
+    for (int i = 0; i < 3; ++i)
+        if (num_given_bounds >= i + 1) {
+            grid_phys_mins[i] = given_mins[i];
+            grid_phys_maxs[i] = given_maxs[i];
+        }
+    for (int i = 0; i < 3; ++i)
+        grid_step_size[i] =
+            (grid_phys_maxs[i] - grid_phys_mins[i]) / (float)(glo_num_idx[i] - 1);
+}
+
+/* BlockGridParams(): src/dense.cpp:575-648 (asserts omitted: the reference
+ * is built as Release, CMakeLists.txt:18-22). */
+void o_block_grid_params(const float *bmin, const float *bmax, int *block_min_idx,
+                         int *block_max_idx, int *block_num_idx,
+                         const float *gmin, const float *step, float eps,
+                         const float *data_mins, const float *data_maxs,
+                         const int *glo_num_idx)
+{
+    float pos[3];
+    o_phys2idx(bmin, block_min_idx, step, gmin);
+    o_idx2phys(block_min_idx, pos, step, gmin);
+    for (int i = 0; i < 3; ++i)
+        if (pos[i] < bmin[i])
+            block_min_idx[i]++;
+    o_phys2idx(bmax, block_max_idx, step, gmin);
+    o_idx2phys(block_max_idx, pos, step, gmin);
+    for (int i = 0; i < 3; ++i)
+        if (pos[i] + step[i] <= bmax[i])
+            block_max_idx[i]++;
+    /* duplicate at the shared max face (note: uses data_mins, :615-626) */
+    for (int i = 0; i < 3; ++i)
+        if (fabsf(data_mins[i] + (float)block_max_idx[i] * step[i] - bmax[i]) < eps &&
+            fabsf(bmax[i] - data_maxs[i]) > step[i])
+            block_max_idx[i]--;
+    for (int i = 0; i < 3; ++i)
+        if (fabsf(bmin[i] - data_mins[i]) < step[i])
+            block_min_idx[i] = 0;
+    for (int i = 0; i < 3; ++i)
+        if (fabsf(bmax[i] - data_maxs[i]) < step[i])
+            block_max_idx[i] = glo_num_idx[i] - 1;
+    for (int i = 0; i < 3; ++i)
+        block_num_idx[i] = block_max_idx[i] - block_min_idx[i] + 1;
+}
+
+/* DistributeScalarCIC(): src/dense.cpp:1787-1872.  Weight of a corner is
+ * v0/vol of the box between the (nudged) point and that corner, normalised;
+ * corners in z, y, x order with x fastest. */
+void o_distribute_cic(const float *pt, float scalar, int *idx24, float *mass8,
+                      const float *step, const float *gmin, float eps)
+{
+    int mn[3];
+    o_phys2idx(pt, mn, step, gmin);
+    float tot = 0.0f, v0 = 0.0f, w[8];
+    int n = 0, ijk[3];
+    for (ijk[2] = mn[2]; ijk[2] <= mn[2] + 1; ijk[2]++)
+        for (ijk[1] = mn[1]; ijk[1] <= mn[1] + 1; ijk[1]++)
+            for (ijk[0] = mn[0]; ijk[0] <= mn[0] + 1; ijk[0]++) {
+                idx24[3 * n + 0] = ijk[0];
+                idx24[3 * n + 1] = ijk[1];
+                idx24[3 * n + 2] = ijk[2];
+                float gp[3], p[3] = {pt[0], pt[1], pt[2]};
+                o_idx2phys(ijk, gp, step, gmin);
+                for (int i = 0; i < 3; ++i)
+                    if (fabsf(p[i] - gp[i]) < eps)
+                        p[i] += 2.0f * eps;
+                float vol = fabsf((gp[0] - p[0]) * (gp[1] - p[1]) * (gp[2] - p[2]));
+                if ((double)v0 == 0.0)
+                    v0 = vol;
+                float wv = v0 / vol;
+                w[n] = wv;
+                tot += wv;
+                n++;
+            }
+    for (int i = 0; i < 8; ++i) {
+        w[i] /= tot;
+        mass8[i] = w[i] * scalar;
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* CellBounds(): src/dense.cpp:657-736                                       */
+/* ------------------------------------------------------------------------ */
+static float *g_fv;     /* ring vertex scratch */
+static int g_fv_cap;
+
+int o_cell_bounds(const o_block_t *b, int cell, float *cmin, float *cmax,
+                  float *normals, float *f0, int cap)
+{
+    const o_tet_t *tets = b->tets;
+    int t = b->vert_to_tet[cell];
+    star_bfs(cell, tets, t, &g_q, &g_vis, &g_vv, &g_nu, &g_nt);
+    int nf = g_nu.n;
+    if (nf > cap)
+        return -1;
+    for (int k = 0; k < nf; k++) {
+        int u = g_nu.p[k];
+        int ut = g_nt.p[k];
+        edge_link(&g_link, cell, u, ut, tets);
+        int L = g_link.n;
+        if (3 * L > g_fv_cap) {
+            g_fv_cap = 3 * L + 64;
+            g_fv = (float *)realloc(g_fv, sizeof(float) * (size_t)g_fv_cap);
+        }
+        for (int l = 0; l < L; l++) {
+            float vv[3];
+            o_circumcenter(vv, &tets[g_link.p[l]], b->particles);
+            g_fv[3 * l + 0] = vv[0];
+            g_fv[3 * l + 1] = vv[1];
+            g_fv[3 * l + 2] = vv[2];
+            for (int i = 0; i < 3; ++i) {
+                if ((k == 0 && l == 0) || vv[i] < cmin[i])
+                    cmin[i] = vv[i];
+                if ((k == 0 && l == 0) || vv[i] > cmax[i])
+                    cmax[i] = vv[i];
+            }
+        }
+        float n[3];
+        o_newell_normal(g_fv, L, n);
+        const float *site = &b->particles[3 * cell];
+        float w[3] = {g_fv[0] - site[0], g_fv[1] - site[1], g_fv[2] - site[2]};
+        if ((double)(w[0] * n[0] + w[1] * n[1] + w[2] * n[2]) < 0.0) {
+            n[0] = -n[0];
+            n[1] = -n[1];
+            n[2] = -n[2];
+        }
+        normals[3 * k + 0] = n[0];
+        normals[3 * k + 1] = n[1];
+        normals[3 * k + 2] = n[2];
+        f0[3 * k + 0] = g_fv[0];
+        f0[3 * k + 1] = g_fv[1];
+        f0[3 * k + 2] = g_fv[2];
+    }
+    return nf;
+}
+
+/* ------------------------------------------------------------------------ */
+/* CellInteriorGridPts(): the scanline search, src/dense.cpp:1475-1701.      */
+/* Returns the per-line runs in border[] and the total point count.          */
+/* ------------------------------------------------------------------------ */
+static int *g_border;
+static long g_border_cap;
+
+static int interior_scan(const int *cgp, const float *cmin_pos,
+                         const float *normals, const float *f0, int nf,
+                         const float *step, float eps)
+{
+    long need = 2L * cgp[1] * cgp[2];
+    if (need > g_border_cap) {
+        g_border_cap = need + 64;
+        g_border = (int *)realloc(g_border, sizeof(int) * (size_t)g_border_cap);
+    }
+    int *border = g_border;
+    int tot = 0;
+    float gp[3];
+    int x_left = cgp[0] / 2, x_right = cgp[0] / 2;
+    int y_start = 0;
+    int x_in_left, x_in_right, min_xi, max_xi;
+    int border_found, z_step_done;
+    int xi, yi, zi, yj, first_x = 0;
+
+    for (zi = 0; zi < cgp[2]; zi++) {
+        gp[2] = cmin_pos[2] + (float)zi * step[2];
+        border_found = 0;
+        z_step_done = 0;
+        for (yj = 0; yj < y_start; yj++) {
+            border[2 * (zi * cgp[1] + yj)] = 1;
+            border[2 * (zi * cgp[1] + yj) + 1] = 0;
+        }
+        for (yi = y_start; yi < cgp[1]; yi++) {
+            gp[1] = cmin_pos[1] + (float)yi * step[1];
+            /* both stepping flags come from the point at x_left (:1530-1542) */
+            gp[0] = cmin_pos[0] + (float)x_left * step[0];
+            gp[2] = cmin_pos[2] + (float)zi * step[2];
+            if (o_pt_in_cell(gp, normals, f0, nf, eps)) {
+                x_in_left = 1;
+                x_in_right = 1;
+            } else {
+                x_in_left = 0;
+                x_in_right = 0;
+            }
+            min_xi = cgp[0] - 1;
+            max_xi = 0;
+
+            /* x step left (:1547-1582) */
+            for (xi = x_left; xi >= 0 && xi < cgp[0];) {
+                gp[0] = cmin_pos[0] + (float)xi * step[0];
+                if (o_pt_in_cell(gp, normals, f0, nf, eps)) {
+                    if (xi < min_xi) min_xi = xi;
+                    if (xi > max_xi) max_xi = xi;
+                    if (x_in_left) {
+                        xi--;
+                    } else {
+                        x_left = xi;
+                        break;
+                    }
+                } else {
+                    if (!x_in_left) {
+                        xi++;
+                    } else {
+                        x_left = xi;
+                        break;
+                    }
+                }
+            }
+            /* x step right (:1585-1620) */
+            for (xi = x_right; xi >= 0 && xi < cgp[0];) {
+                gp[0] = cmin_pos[0] + (float)xi * step[0];
+                if (o_pt_in_cell(gp, normals, f0, nf, eps)) {
+                    if (xi < min_xi) min_xi = xi;
+                    if (xi > max_xi) max_xi = xi;
+                    if (x_in_right) {
+                        xi++;
+                    } else {
+                        x_right = xi;
+                        break;
+                    }
+                } else {
+                    if (!x_in_right) {
+                        xi--;
+                    } else {
+                        x_right = xi;
+                        break;
+                    }
+                }
+            }
+            border[2 * (zi * cgp[1] + yi)] = min_xi;
+            border[2 * (zi * cgp[1] + yi) + 1] = max_xi;
+
+            if (min_xi <= max_xi) {
+                tot += (max_xi - min_xi + 1);
+                if (yi == y_start)
+                    first_x = (min_xi + max_xi) / 2;
+            }
+            int first_y = y_start;
+            if (min_xi <= max_xi && !border_found) {
+                first_y = yi;
+                border_found = 1;
+            }
+            if (min_xi > max_xi && border_found)
+                z_step_done = 1;
+
+            /* loosen y_start for the next z step (:1649-1661) */
+            if ((yi == cgp[1] - 1 || z_step_done) && zi + 1 < cgp[2]) {
+                gp[2] = cmin_pos[2] + (float)(zi + 1) * step[2];
+                gp[0] = cmin_pos[0] + (float)first_x * step[0];
+                for (yj = first_y; yj > 0; yj--) {
+                    gp[1] = cmin_pos[1] + (float)yj * step[1];
+                    if (!o_pt_in_cell(gp, normals, f0, nf, eps))
+                        break;
+                }
+                y_start = yj;
+            }
+            if (z_step_done) {
+                for (yj = yi + 1; yj < cgp[1]; yj++) {
+                    border[2 * (zi * cgp[1] + yj)] = 1;
+                    border[2 * (zi * cgp[1] + yj) + 1] = 0;
+                }
+                break;
+            }
+        }
+    }
+    return tot;
+}
+
+/* ------------------------------------------------------------------------ */
+/* per-cell point list: IterateCells body + CellGridPts                     */
+/* ------------------------------------------------------------------------ */
+static float *g_norm, *g_f0;
+static int g_face_cap;
+
+static int ensure_faces(int nf)
+{
+    if (nf > g_face_cap) {
+        g_face_cap = nf + 64;
+        g_norm = (float *)realloc(g_norm, sizeof(float) * 3 * (size_t)g_face_cap);
+        g_f0 = (float *)realloc(g_f0, sizeof(float) * 3 * (size_t)g_face_cap);
+    }
+    return g_face_cap;
+}
+
+/* Computes the cell's points into (idx3, mass); returns count or -1 when
+ * cap is exceeded.  Mirrors src/dense.cpp:250-268 then :1381-1457. */
+int o_cell_points(const o_block_t *b, const o_args_t *a, int cell, int *idx3,
+                  double *mass, int cap, int *status)
+{
+    int vt = b->vert_to_tet[cell];
+    if (vt == -1 || !o_complete(cell, b->tets, b->num_tets, vt)) {
+        *status = O_CELL_INCOMPLETE;
+        return 0;
+    }
+    /* CellBounds: the face count is the BFS neighbour count */
+    int nf;
+    for (;;) {
+        float cmin[3], cmax[3];
+        nf = o_cell_bounds(b, cell, cmin, cmax, g_norm, g_f0, g_face_cap);
+        if (nf < 0) {
+            ensure_faces(g_nu.n);
+            continue;
+        }
+        /* CellGridPts: data-bounds filter (:1381-1392) */
+        float epsv[3];
+        for (int i = 0; i < 3; ++i)
+            epsv[i] = (a->data_maxs[i] - a->data_mins[i]) * 2.0f * FLT_EPSILON;
+        if (cmin[0] < a->data_mins[0] - epsv[0] || cmin[1] < a->data_mins[1] - epsv[1] ||
+            cmin[2] < a->data_mins[2] - epsv[2] || cmax[0] > a->data_maxs[0] + epsv[0] ||
+            cmax[1] > a->data_maxs[1] + epsv[1] || cmax[2] > a->data_maxs[2] + epsv[2]) {
+            *status = O_CELL_OUTSIDE;
+            return 0;
+        }
+        if (isnan(cmin[0]) || isnan(cmin[1]) || isnan(cmin[2]) || isnan(cmax[0]) ||
+            isnan(cmax[1]) || isnan(cmax[2])) {
+            /* the reference's phys2idx of NaN is undefined; skip the cell */
+            *status = O_CELL_DEGENERATE;
+            return 0;
+        }
+        int cmi[3], cma[3], cgp[3];
+        float cminpos[3];
+        o_phys2idx(cmin, cmi, a->grid_step_size, a->grid_phys_mins);
+        o_phys2idx(cmax, cma, a->grid_step_size, a->grid_phys_mins);
+        o_idx2phys(cmi, cminpos, a->grid_step_size, a->grid_phys_mins);
+        for (int i = 0; i < 3; ++i)
+            cgp[i] = cma[i] - cmi[i] + 1;
+        int tot = interior_scan(cgp, cminpos, g_norm, g_f0, nf, a->grid_step_size,
+                                a->eps);
+        if (tot > 0) {
+            if (tot > cap)
+                return -1;
+            float mpp = a->mass / (float)tot;      /* :1692 */
+            int n = 0;
+            for (int zi = 0; zi < cgp[2]; zi++)
+                for (int yi = 0; yi < cgp[1]; yi++) {
+                    int lo = g_border[2 * (zi * cgp[1] + yi)];
+                    int hi = g_border[2 * (zi * cgp[1] + yi) + 1];
+                    for (int xi = lo; xi <= hi; xi++) {
+                        idx3[3 * n + 0] = cmi[0] + xi;
+                        idx3[3 * n + 1] = cmi[1] + yi;
+                        idx3[3 * n + 2] = cmi[2] + zi;
+                        mass[n] = (double)mpp;
+                        n++;
+                    }
+                }
+            *status = O_CELL_SCAN;
+            return n;
+        }
+        if (cap < 8)
+            return -1;
+        int i24[24];
+        float m8[8];
+        o_distribute_cic(&b->particles[3 * cell], a->mass, i24, m8,
+                         a->grid_step_size, a->grid_phys_mins, a->eps);
+        for (int i = 0; i < 8; ++i) {
+            idx3[3 * i + 0] = i24[3 * i + 0];
+            idx3[3 * i + 1] = i24[3 * i + 1];
+            idx3[3 * i + 2] = i24[3 * i + 2];
+            mass[i] = (double)m8[i];
+        }
+        *status = O_CELL_CIC;
+        return 8;
+    }
+}
+
+/* deposit one point (src/dense.cpp:276-305): in-block -> density,
+ * else -> off-block list.  Returns 1 if off-block. */
+static int deposit(const o_block_t *b, const o_args_t *a, const int *bmin_idx,
+                   const int *bnum_idx, float *density, const int *gidx,
+                   double m, float div, double *tot_mass)
+{
+    float pos[3];
+    o_idx2phys(gidx, pos, a->grid_step_size, a->grid_phys_mins);
+    if (pos[0] >= b->bounds_min[0] && pos[0] <= b->bounds_max[0] &&
+        pos[1] >= b->bounds_min[1] && pos[1] <= b->bounds_max[1] &&
+        pos[2] >= b->bounds_min[2] && pos[2] <= b->bounds_max[2]) {
+        int l[3] = {gidx[0] - bmin_idx[0], gidx[1] - bmin_idx[1],
+                    gidx[2] - bmin_idx[2]};
+        int idx = o_index(l, bnum_idx, a->project, a->proj_plane);
+        density[idx] = (float)((double)density[idx] + m / (double)div);
+        if (tot_mass)
+            *tot_mass += m;
+        return 0;
+    }
+    return 1;
+}
+
+static int *g_pidx;
+static double *g_pmass;
+static int g_pcap;
+
+long o_iterate_cells(const o_block_t *b, const o_args_t *a,
+                     const int *block_min_idx, const int *block_num_idx,
+                     float *density, int *cell_status, int *cell_npts,
+                     int *off_idx3, double *off_mass, int *off_cell,
+                     long off_cap, double *tot_mass)
+{
+    float div = a->project ? a->grid_step_size[0] * a->grid_step_size[1]
+                           : a->grid_step_size[0] * a->grid_step_size[1] *
+                                 a->grid_step_size[2];
+    long noff = 0;
+    ensure_faces(64);
+    if (!g_pcap) {
+        g_pcap = 4096;
+        g_pidx = (int *)malloc(sizeof(int) * 3 * (size_t)g_pcap);
+        g_pmass = (double *)malloc(sizeof(double) * (size_t)g_pcap);
+    }
+    for (int cell = 0; cell < b->num_orig_particles; cell++) {
+        int st, n;
+        while ((n = o_cell_points(b, a, cell, g_pidx, g_pmass, g_pcap, &st)) < 0) {
+            g_pcap *= 4;
+            g_pidx = (int *)realloc(g_pidx, sizeof(int) * 3 * (size_t)g_pcap);
+            g_pmass = (double *)realloc(g_pmass, sizeof(double) * (size_t)g_pcap);
+        }
+        if (cell_status)
+            cell_status[cell] = st;
+        if (cell_npts)
+            cell_npts[cell] = n;
+        for (int i = 0; i < n; i++) {
+            if (deposit(b, a, block_min_idx, block_num_idx, density, &g_pidx[3 * i],
+                        g_pmass[i], div, tot_mass)) {
+                if (noff < off_cap) {
+                    off_idx3[3 * noff + 0] = g_pidx[3 * i + 0];
+                    off_idx3[3 * noff + 1] = g_pidx[3 * i + 1];
+                    off_idx3[3 * noff + 2] = g_pidx[3 * i + 2];
+                    off_mass[noff] = g_pmass[i];
+                    off_cell[noff] = cell;
+                }
+                noff++;
+            }
+        }
+    }
+    return noff;
+}
+
+long o_iterate_cells_cic(const o_block_t *b, const o_args_t *a,
+                         const int *block_min_idx, const int *block_num_idx,
+                         float *density, int *off_idx3, double *off_mass,
+                         int *off_cell, long off_cap, double *tot_mass)
+{
+    float div = a->project ? a->grid_step_size[0] * a->grid_step_size[1]
+                           : a->grid_step_size[0] * a->grid_step_size[1] *
+                                 a->grid_step_size[2];
+    long noff = 0;
+    for (int cell = 0; cell < b->num_orig_particles; cell++) {
+        int i24[24];
+        float m8[8];
+        o_distribute_cic(&b->particles[3 * cell], a->mass, i24, m8,
+                         a->grid_step_size, a->grid_phys_mins, a->eps);
+        for (int i = 0; i < 8; i++) {
+            if (deposit(b, a, block_min_idx, block_num_idx, density, &i24[3 * i],
+                        (double)m8[i], div, tot_mass)) {
+                if (noff < off_cap) {
+                    off_idx3[3 * noff + 0] = i24[3 * i + 0];
+                    off_idx3[3 * noff + 1] = i24[3 * i + 1];
+                    off_idx3[3 * noff + 2] = i24[3 * i + 2];
+                    off_mass[noff] = (double)m8[i];
+                    off_cell[noff] = cell;
+                }
+                noff++;
+            }
+        }
+    }
+    return noff;
+}
+
+void o_recvd_pts(const int *idx3, const double *mass, long n,
+                 const int *block_min_idx, const int *block_num_idx,
+                 const o_args_t *a, float *density)
+{
+    float div = a->project ? a->grid_step_size[0] * a->grid_step_size[1]
+                           : a->grid_step_size[0] * a->grid_step_size[1] *
+                                 a->grid_step_size[2];
+    for (long j = 0; j < n; j++) {
+        int l[3] = {idx3[3 * j] - block_min_idx[0], idx3[3 * j + 1] - block_min_idx[1],
+                    idx3[3 * j + 2] - block_min_idx[2]};
+        int idx = o_index(l, block_num_idx, a->project, a->proj_plane);
+        density[idx] = (float)((double)density[idx] + mass[j] / (double)div);
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* exact Voronoi owner (FP64 oracle)                                         */
+/* ------------------------------------------------------------------------ */
+static double d2_site(const float *g, const float *p)
+{
+    double dx = (double)g[0] - (double)p[0];
+    double dy = (double)g[1] - (double)p[1];
+    double dz = (double)g[2] - (double)p[2];
+    return dx * dx + dy * dy + dz * dz;
+}
+
+void o_exact_owner(const float *particles, int np, const int *gnum,
+                   const float *step, const float *gmin, int *owner)
+{
+    /* bucket hash over the particle bounding box, ~2 particles per bucket */
+    double lo[3], hi[3];
+    for (int i = 0; i < 3; ++i) {
+        lo[i] = 1e300;
+        hi[i] = -1e300;
+    }
+    for (int p = 0; p < np; ++p)
+        for (int i = 0; i < 3; ++i) {
+            double x = particles[3 * p + i];
+            if (x < lo[i]) lo[i] = x;
+            if (x > hi[i]) hi[i] = x;
+        }
+    double ext = 0.0;
+    for (int i = 0; i < 3; ++i)
+        if (hi[i] - lo[i] > ext) ext = hi[i] - lo[i];
+    if (ext <= 0.0) ext = 1.0;
+    int nb = (int)cbrt((double)np / 2.0);
+    if (nb < 1) nb = 1;
+    double h = ext / nb * (1.0 + 1e-9);
+    int dims[3];
+    for (int i = 0; i < 3; ++i) {
+        dims[i] = (int)((hi[i] - lo[i]) / h) + 1;
+    }
+    long nbk = (long)dims[0] * dims[1] * dims[2];
+    int *start = (int *)calloc((size_t)nbk + 1, sizeof(int));
+    int *items = (int *)malloc(sizeof(int) * (size_t)np);
+    int *bk = (int *)malloc(sizeof(int) * (size_t)np);
+    for (int p = 0; p < np; ++p) {
+        int c[3];
+        for (int i = 0; i < 3; ++i) {
+            c[i] = (int)(((double)particles[3 * p + i] - lo[i]) / h);
+            if (c[i] >= dims[i]) c[i] = dims[i] - 1;
+        }
+        bk[p] = (c[2] * dims[1] + c[1]) * dims[0] + c[0];
+        start[bk[p] + 1]++;
+    }
+    for (long i = 0; i < nbk; ++i)
+        start[i + 1] += start[i];
+    int *fill = (int *)malloc(sizeof(int) * (size_t)nbk);
+    memcpy(fill, start, sizeof(int) * (size_t)nbk);
+    for (int p = 0; p < np; ++p)       /* ascending id within each bucket */
+        items[fill[bk[p]]++] = p;
+    free(fill);
+    free(bk);
+
+    for (int z = 0; z < gnum[2]; ++z)
+        for (int y = 0; y < gnum[1]; ++y)
+            for (int x = 0; x < gnum[0]; ++x) {
+                int gi[3] = {x, y, z};
+                float g[3];
+                o_idx2phys(gi, g, step, gmin);
+                int c[3];
+                for (int i = 0; i < 3; ++i) {
+                    double f = ((double)g[i] - lo[i]) / h;
+                    int ci = (int)floor(f);
+                    if (ci < 0) ci = 0;
+                    if (ci >= dims[i]) ci = dims[i] - 1;
+                    c[i] = ci;
+                }
+                double best = 1e300;
+                int bid = -1;
+                for (int r = 0;; ++r) {
+                    /* shell r around c: buckets with Chebyshev distance r */
+                    int any = 0;
+                    for (int dz = -r; dz <= r; ++dz)
+                        for (int dy = -r; dy <= r; ++dy)
+                            for (int dx = -r; dx <= r; ++dx) {
+                                if (abs(dx) != r && abs(dy) != r && abs(dz) != r)
+                                    continue;
+                                int q[3] = {c[0] + dx, c[1] + dy, c[2] + dz};
+                                if (q[0] < 0 || q[1] < 0 || q[2] < 0 || q[0] >= dims[0] ||
+                                    q[1] >= dims[1] || q[2] >= dims[2])
+                                    continue;
+                                any = 1;
+                                long bb = ((long)q[2] * dims[1] + q[1]) * dims[0] + q[0];
+                                for (int s = start[bb]; s < start[bb + 1]; ++s) {
+                                    int p = items[s];
+                                    double d = d2_site(g, &particles[3 * p]);
+                                    if (d < best || (d == best && p < bid)) {
+                                        best = d;
+                                        bid = p;
+                                    }
+                                }
+                            }
+                    /* every bucket beyond shell r lies at least r*h away
+                     * (g may sit outside the hash box: then distances are
+                     * only larger); stop once that exceeds the best */
+                    double lb = (double)r * h;
+                    if (bid >= 0 && lb * lb > best * (1.0 + 1e-9) + 1e-30)
+                        break;
+                    if (!any && r > dims[0] + dims[1] + dims[2])
+                        break;
+                }
+                owner[((long)z * gnum[1] + y) * gnum[0] + x] = bid;
+            }
+    free(start);
+    free(items);
+}

@@ -1,0 +1,661 @@
+// api.hip -- the C ABI (include/tess2_amd.h): context, staging of blocks in
+// HBM, the run sequence on one HIP stream, and the reference's host-side
+// grid helpers.
+#include "../../include/tess2_amd.h"
+#include "geom.hpp"
+#include "kernels.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+using namespace t2;
+
+namespace {
+
+struct DevBlock {
+    int gid = 0, norig = 0, np = 0, nt = 0;
+    float bmin[3] = {0, 0, 0}, bmax[3] = {0, 0, 0};
+    float4 *pts = nullptr;
+    int4 *tets = nullptr;
+    int *v2t = nullptr;
+    float4 *cc = nullptr;
+    float *vol = nullptr;
+    int *status = nullptr, *npts = nullptr;
+    // grid
+    GridP gp{};
+    size_t ngrid = 0, grid_cap = 0;
+    float *density = nullptr;
+    unsigned *count = nullptr;
+    uint2 *first = nullptr;
+    int *head = nullptr;
+    int4 *ovf = nullptr;
+    unsigned ovf_cap = 0;
+    int *slow = nullptr;
+    unsigned slow_cap = 0;
+    unsigned *ctrs = nullptr;                // C_NCTR
+    unsigned long long *cstat = nullptr;     // 6 cell stats
+    double *gsum = nullptr;
+    unsigned *gmax = nullptr;
+    // host mirrors
+    unsigned h_ctrs[C_NCTR] = {};
+    unsigned long long h_cstat[6] = {};
+    double h_gsum = 0.0;
+    unsigned h_gmax = 0;
+    int block_max_idx[3] = {0, 0, 0};
+};
+
+struct RunArgs {
+    int alg = 0, num_given = 0, project = 0, assignment = 0, precision = 0, want_vol = 1;
+    float given_mins[3] = {0, 0, 0}, given_maxs[3] = {0, 0, 0};
+    float mass = 1.0f, eps = 1e-4f;
+    int gnum[3] = {0, 0, 0};
+};
+
+}  // namespace
+
+struct tess2_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    std::vector<DevBlock> blocks;
+    hipEvent_t ev_run0 = nullptr, ev_run1 = nullptr, ev_k0 = nullptr, ev_k1 = nullptr;
+    RunArgs last;
+    bool have_run = false;
+    float data_mins[3] = {0, 0, 0}, data_maxs[3] = {0, 0, 0};
+    float gmin[3] = {0, 0, 0}, gmax[3] = {0, 0, 0}, step[3] = {0, 0, 0};
+    double ktime_sum = 0.0;
+    int ktime_n = 0;
+    float last_run_ms = 0.0f;
+};
+
+static int fail(tess2_ctx *c, int code, const char *fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    if (c) c->err = buf;
+    return code;
+}
+
+#define HIPCHK(c, x)                                                                   \
+    do {                                                                               \
+        hipError_t e_ = (x);                                                           \
+        if (e_ != hipSuccess)                                                          \
+            return fail((c), TESS2_E_HIP, "%s failed: %s", #x, hipGetErrorString(e_)); \
+    } while (0)
+
+template <class T>
+static int dalloc(tess2_ctx *c, T **p, size_t n)
+{
+    if (*p) {
+        (void)hipFree(*p);
+        *p = nullptr;
+    }
+    if (n == 0) n = 1;
+    hipError_t e = hipMalloc((void **)p, n * sizeof(T));
+    if (e != hipSuccess) {
+        *p = nullptr;
+        return fail(c, TESS2_E_NOMEM, "hipMalloc(%zu bytes): %s", n * sizeof(T),
+                    hipGetErrorString(e));
+    }
+    return 0;
+}
+
+static void free_block(DevBlock &b)
+{
+    void *ps[] = {b.pts, b.tets, b.v2t, b.cc, b.vol, b.status, b.npts, b.density, b.count,
+                  b.first, b.head, b.ovf, b.slow, b.ctrs, b.cstat, b.gsum, b.gmax};
+    for (void *p : ps)
+        if (p) (void)hipFree(p);
+    b = DevBlock();
+}
+
+// ---------------------------------------------------------------------------
+// host helpers: GridStepParams / BlockGridParams (same arithmetic as the
+// reference, see geom.hpp for the float conventions)
+// ---------------------------------------------------------------------------
+extern "C" void tess2_grid_step_params(int num_given_bounds, const float *given_mins,
+                                       const float *given_maxs, const float *data_mins,
+                                       const float *data_maxs, float *gmin, float *gmax,
+                                       float *step, const int *gnum)
+{
+    // src/dense.cpp:1721-1766; the padding is divided by the double 2.0
+    float mds = data_maxs[0] - data_mins[0];
+    if (data_maxs[1] - data_mins[1] > mds) mds = data_maxs[1] - data_mins[1];
+    if (data_maxs[2] - data_mins[2] > mds) mds = data_maxs[2] - data_mins[2];
+    for (int i = 0; i < 3; ++i) {
+        const float pad = mds - (data_maxs[i] - data_mins[i]);
+        gmin[i] = (float)((double)data_mins[i] - (double)pad / 2.0);
+        gmax[i] = (float)((double)data_maxs[i] + (double)pad / 2.0);
+    }
+    for (int i = 0; i < 3; ++i)
+        if (num_given_bounds >= i + 1) {
+            gmin[i] = given_mins[i];
+            gmax[i] = given_maxs[i];
+        }
+    for (int i = 0; i < 3; ++i) step[i] = (gmax[i] - gmin[i]) / (float)(gnum[i] - 1);
+}
+
+extern "C" void tess2_block_grid_params(const float *bmin, const float *bmax, int *mn,
+                                        int *mx, int *num, const float *gmin,
+                                        const float *step, float eps, const float *data_mins,
+                                        const float *data_maxs, const int *gnum)
+{
+    // src/dense.cpp:586-647
+    for (int i = 0; i < 3; ++i) {
+        mn[i] = phys2idx1(bmin[i], step[i], gmin[i]);
+        if (idx2phys1(mn[i], step[i], gmin[i]) < bmin[i]) mn[i]++;
+        mx[i] = phys2idx1(bmax[i], step[i], gmin[i]);
+        if (idx2phys1(mx[i], step[i], gmin[i]) + step[i] <= bmax[i]) mx[i]++;
+    }
+    for (int i = 0; i < 3; ++i)
+        if (std::fabs(data_mins[i] + (float)mx[i] * step[i] - bmax[i]) < eps &&
+            std::fabs(bmax[i] - data_maxs[i]) > step[i])
+            mx[i]--;
+    for (int i = 0; i < 3; ++i)
+        if (std::fabs(bmin[i] - data_mins[i]) < step[i]) mn[i] = 0;
+    for (int i = 0; i < 3; ++i)
+        if (std::fabs(bmax[i] - data_maxs[i]) < step[i]) mx[i] = gnum[i] - 1;
+    for (int i = 0; i < 3; ++i) num[i] = mx[i] - mn[i] + 1;
+}
+
+// ---------------------------------------------------------------------------
+extern "C" int tess2_create(tess2_ctx **out, int device)
+{
+    if (!out) return TESS2_E_ARG;
+    *out = nullptr;
+    tess2_ctx *c = new tess2_ctx();
+    c->device = device;
+    hipError_t e = hipSetDevice(device);
+    if (e != hipSuccess) {
+        delete c;
+        return TESS2_E_HIP;
+    }
+    // default priority, non-blocking w.r.t. the null stream
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&c->ev_run0) != hipSuccess || hipEventCreate(&c->ev_run1) != hipSuccess ||
+        hipEventCreate(&c->ev_k0) != hipSuccess || hipEventCreate(&c->ev_k1) != hipSuccess) {
+        delete c;
+        return TESS2_E_HIP;
+    }
+    *out = c;
+    return 0;
+}
+
+extern "C" int tess2_destroy(tess2_ctx *c)
+{
+    if (!c) return 0;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (auto &b : c->blocks) free_block(b);
+    if (c->ev_run0) (void)hipEventDestroy(c->ev_run0);
+    if (c->ev_run1) (void)hipEventDestroy(c->ev_run1);
+    if (c->ev_k0) (void)hipEventDestroy(c->ev_k0);
+    if (c->ev_k1) (void)hipEventDestroy(c->ev_k1);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return 0;
+}
+
+extern "C" const char *tess2_last_error(const tess2_ctx *c)
+{
+    return c ? c->err.c_str() : "null context";
+}
+
+// ---------------------------------------------------------------------------
+// staging
+// ---------------------------------------------------------------------------
+static int validate(tess2_ctx *c, const tess2_block_t &b)
+{
+    if (b.num_orig_particles < 0 || b.num_particles < b.num_orig_particles || b.num_tets < 0)
+        return fail(c, TESS2_E_ARG, "block %d: bad counts", b.gid);
+    if ((b.num_particles && !b.particles) || (b.num_tets && !b.tets) ||
+        (b.num_particles && !b.vert_to_tet))
+        return fail(c, TESS2_E_ARG, "block %d: null array", b.gid);
+    const int np = b.num_particles, nt = b.num_tets;
+    for (int t = 0; t < nt; ++t) {
+        const tess2_tet_t &q = b.tets[t];
+        for (int i = 0; i < 4; ++i) {
+            if (q.verts[i] < 0 || q.verts[i] >= np)
+                return fail(c, TESS2_E_ARG, "block %d: tet %d vertex %d out of range", b.gid, t,
+                            q.verts[i]);
+            if (q.tets[i] < -1 || q.tets[i] >= nt)
+                return fail(c, TESS2_E_ARG, "block %d: tet %d neighbour %d out of range", b.gid,
+                            t, q.tets[i]);
+        }
+    }
+    for (int p = 0; p < np; ++p)
+        if (b.vert_to_tet[p] < -1 || b.vert_to_tet[p] >= nt)
+            return fail(c, TESS2_E_ARG, "block %d: vert_to_tet[%d] out of range", b.gid, p);
+    return 0;
+}
+
+extern "C" int tess2_stage(tess2_ctx *c, const tess2_block_t *blocks, int nblocks)
+{
+    if (!c || (nblocks > 0 && !blocks) || nblocks < 0) return fail(c, TESS2_E_ARG, "bad args");
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (int i = 0; i < nblocks; ++i) {
+        int r = validate(c, blocks[i]);
+        if (r) return r;
+    }
+    for (auto &b : c->blocks) free_block(b);
+    c->blocks.assign(nblocks, DevBlock());
+    c->have_run = false;
+    std::vector<float4> tmp;
+    for (int k = 0; k < nblocks; ++k) {
+        const tess2_block_t &hb = blocks[k];
+        DevBlock &d = c->blocks[k];
+        d.gid = hb.gid;
+        d.norig = hb.num_orig_particles;
+        d.np = hb.num_particles;
+        d.nt = hb.num_tets;
+        for (int i = 0; i < 3; ++i) {
+            d.bmin[i] = hb.bounds_min[i];
+            d.bmax[i] = hb.bounds_max[i];
+        }
+        int r = 0;
+        if ((r = dalloc(c, &d.pts, (size_t)d.np)) || (r = dalloc(c, &d.tets, 2 * (size_t)d.nt)) ||
+            (r = dalloc(c, &d.v2t, (size_t)d.np)) || (r = dalloc(c, &d.cc, (size_t)d.nt)) ||
+            (r = dalloc(c, &d.vol, (size_t)d.norig)) || (r = dalloc(c, &d.status, (size_t)d.norig)) ||
+            (r = dalloc(c, &d.npts, (size_t)d.norig)) || (r = dalloc(c, &d.ctrs, (size_t)C_NCTR)) ||
+            (r = dalloc(c, &d.cstat, 6)) || (r = dalloc(c, &d.gsum, 1)) || (r = dalloc(c, &d.gmax, 1)))
+            return r;
+        tmp.resize(d.np);
+        for (int p = 0; p < d.np; ++p)
+            tmp[p] = make_float4(hb.particles[3 * p], hb.particles[3 * p + 1],
+                                 hb.particles[3 * p + 2], 0.0f);
+        HIPCHK(c, hipMemcpy(d.pts, tmp.data(), sizeof(float4) * d.np, hipMemcpyHostToDevice));
+        HIPCHK(c, hipMemcpy(d.tets, hb.tets, sizeof(tess2_tet_t) * (size_t)d.nt,
+                            hipMemcpyHostToDevice));
+        HIPCHK(c, hipMemcpy(d.v2t, hb.vert_to_tet, sizeof(int) * (size_t)d.np,
+                            hipMemcpyHostToDevice));
+        d.slow_cap = (unsigned)std::max(1024, d.norig / 32);
+        if ((r = dalloc(c, &d.slow, d.slow_cap))) return r;
+    }
+    return 0;
+}
+
+// grid + list buffers for one run
+static int ensure_grid(tess2_ctx *c, DevBlock &d, size_t n)
+{
+    d.ngrid = n;
+    if (n > d.grid_cap) {
+        int r = 0;
+        if ((r = dalloc(c, &d.density, n)) || (r = dalloc(c, &d.count, n)) ||
+            (r = dalloc(c, &d.first, n)) || (r = dalloc(c, &d.head, n)))
+            return r;
+        d.grid_cap = n;
+    }
+    if (!d.ovf) {
+        d.ovf_cap = (unsigned)std::max<size_t>(1u << 16, n / 8);
+        int r = dalloc(c, &d.ovf, d.ovf_cap);
+        if (r) return r;
+    }
+    return 0;
+}
+
+static CellOut make_out(DevBlock &d, bool want_vol)
+{
+    CellOut o;
+    o.vol = want_vol ? d.vol : nullptr;
+    o.status = d.status;
+    o.npts = d.npts;
+    o.density = d.density;
+    o.count = d.count;
+    o.first = d.first;
+    o.head = d.head;
+    o.ovf = d.ovf;
+    o.ovf_cap = d.ovf_cap;
+    o.ctrs = d.ctrs;
+    o.slow = d.slow;
+    o.slow_cap = d.slow_cap;
+    return o;
+}
+
+static BlockIn make_in(DevBlock &d)
+{
+    BlockIn in;
+    in.pts = d.pts;
+    in.tets = d.tets;
+    in.v2t = d.v2t;
+    in.cc = d.cc;
+    in.ncells = d.norig;
+    in.ntets = d.nt;
+    return in;
+}
+
+// enqueue the whole path for every staged block
+static int enqueue_run(tess2_ctx *c, const RunArgs &a)
+{
+    if (a.gnum[0] < 2 || a.gnum[1] < 2 || a.gnum[2] < 2)
+        return fail(c, TESS2_E_ARG, "grid needs at least 2 points per axis");
+    if (c->blocks.empty()) return fail(c, TESS2_E_ARG, "no staged blocks");
+    // DataBounds(): min/max over the block bounds (src/dense.cpp:1238-1267)
+    for (size_t k = 0; k < c->blocks.size(); ++k)
+        for (int i = 0; i < 3; ++i) {
+            const DevBlock &b = c->blocks[k];
+            if (k == 0 || b.bmin[i] < c->data_mins[i]) c->data_mins[i] = b.bmin[i];
+            if (k == 0 || b.bmax[i] > c->data_maxs[i]) c->data_maxs[i] = b.bmax[i];
+        }
+    tess2_grid_step_params(a.num_given, a.given_mins, a.given_maxs, c->data_mins, c->data_maxs,
+                           c->gmin, c->gmax, c->step, a.gnum);
+    const float div = a.project ? c->step[0] * c->step[1] : c->step[0] * c->step[1] * c->step[2];
+    hipStream_t s = c->stream;
+    HIPCHK(c, hipEventRecord(c->ev_run0, s));
+    bool first_block = true;
+    for (auto &d : c->blocks) {
+        GridP &gp = d.gp;
+        int mx[3];
+        tess2_block_grid_params(d.bmin, d.bmax, gp.bmin_idx, mx, gp.bnum_idx, c->gmin, c->step,
+                                a.eps, c->data_mins, c->data_maxs, a.gnum);
+        for (int i = 0; i < 3; ++i) {
+            d.block_max_idx[i] = mx[i];
+            gp.gmin[i] = c->gmin[i];
+            gp.step[i] = c->step[i];
+            gp.dmin[i] = c->data_mins[i];
+            gp.dmax[i] = c->data_maxs[i];
+            gp.bmin[i] = d.bmin[i];
+            gp.bmax[i] = d.bmax[i];
+        }
+        if (gp.bnum_idx[0] <= 0 || gp.bnum_idx[1] <= 0 || gp.bnum_idx[2] <= 0)
+            return fail(c, TESS2_E_ARG, "block %d has an empty grid", d.gid);
+        gp.eps = a.eps;
+        gp.mass = a.mass;
+        gp.div = div;
+        gp.project = a.project;
+        const size_t n = (size_t)gp.bnum_idx[0] * gp.bnum_idx[1] *
+                         (a.project ? 1 : (size_t)gp.bnum_idx[2]);
+        int r = ensure_grid(c, d, n);
+        if (r) return r;
+
+        launch_fill_u32(d.ctrs, C_NCTR, 0u, s);
+        launch_fill_u32((unsigned *)d.density, n, 0u, s);
+        launch_fill_u32(d.count, n, 0u, s);
+        launch_fill_u32((unsigned *)d.cstat, 12, 0u, s);
+        launch_fill_u32((unsigned *)d.gsum, 2, 0u, s);
+        launch_fill_u32(d.gmax, 1, 0u, s);
+        BlockIn in = make_in(d);
+        CellOut o = make_out(d, a.want_vol != 0);
+        if (a.alg == TESS2_DENSE_CIC) {
+            if (first_block) HIPCHK(c, hipEventRecord(c->ev_k0, s));
+            launch_cic(in, gp, o, s);
+            if (first_block) HIPCHK(c, hipEventRecord(c->ev_k1, s));
+        } else {
+            launch_circumcenters(d.pts, d.tets, d.nt, d.cc, s);
+            if (first_block) HIPCHK(c, hipEventRecord(c->ev_k0, s));
+            launch_cells_ref(in, gp, o, 1, s);
+            if (first_block) HIPCHK(c, hipEventRecord(c->ev_k1, s));
+            launch_cells_ref_slow(in, gp, o, 1, s);
+        }
+        launch_finalize(o, gp, s);
+        launch_cell_stats(d.status, d.npts, d.norig, d.cstat, s);
+        launch_grid_stats(d.density, n, d.gsum, d.gmax, s);
+        HIPCHK(c, hipGetLastError());
+        HIPCHK(c, hipMemcpyAsync(d.h_ctrs, d.ctrs, sizeof(unsigned) * C_NCTR,
+                                 hipMemcpyDeviceToHost, s));
+        HIPCHK(c, hipMemcpyAsync(d.h_cstat, d.cstat, sizeof(d.h_cstat), hipMemcpyDeviceToHost, s));
+        HIPCHK(c, hipMemcpyAsync(&d.h_gsum, d.gsum, sizeof(double), hipMemcpyDeviceToHost, s));
+        HIPCHK(c, hipMemcpyAsync(&d.h_gmax, d.gmax, sizeof(unsigned), hipMemcpyDeviceToHost, s));
+        first_block = false;
+    }
+    HIPCHK(c, hipEventRecord(c->ev_run1, s));
+    c->last = a;
+    c->have_run = true;
+    return 0;
+}
+
+extern "C" int tess2_run(tess2_ctx *c, int alg, int num_given_bounds, const float *given_mins,
+                         const float *given_maxs, int project, const float *proj_plane,
+                         float mass, float eps, const int *glo_num_idx, int assignment,
+                         int precision, int want_volumes)
+{
+    if (!c || !glo_num_idx) return fail(c, TESS2_E_ARG, "null argument");
+    if (alg != TESS2_DENSE_TESS && alg != TESS2_DENSE_CIC)
+        return fail(c, TESS2_E_ARG, "unknown estimator %d", alg);
+    if (assignment != TESS2_ASSIGN_REFERENCE_SCAN)
+        return fail(c, TESS2_E_ARG, "assignment mode %d not available", assignment);
+    if (project) {
+        // the reference's index() handles the x-y plane only (src/dense.cpp:1076-1077)
+        if (!proj_plane || proj_plane[0] != 0.0f || proj_plane[1] != 0.0f ||
+            proj_plane[2] == 0.0f)
+            return fail(c, TESS2_E_ARG, "projection supports the x-y plane only");
+    }
+    HIPCHK(c, hipSetDevice(c->device));
+    RunArgs a;
+    a.alg = alg;
+    a.num_given = num_given_bounds;
+    for (int i = 0; i < 3; ++i) {
+        a.given_mins[i] = (num_given_bounds > i && given_mins) ? given_mins[i] : 0.0f;
+        a.given_maxs[i] = (num_given_bounds > i && given_maxs) ? given_maxs[i] : 0.0f;
+        a.gnum[i] = glo_num_idx[i];
+    }
+    a.project = project ? 1 : 0;
+    a.mass = mass;
+    a.eps = eps;
+    a.assignment = assignment;
+    a.precision = precision;
+    a.want_vol = want_volumes;
+    return enqueue_run(c, a);
+}
+
+extern "C" int tess2_sync(tess2_ctx *c, tess2_stats_t *st)
+{
+    if (!c) return TESS2_E_ARG;
+    HIPCHK(c, hipSetDevice(c->device));
+    for (int attempt = 0;; ++attempt) {
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        if (!c->have_run) return fail(c, TESS2_E_ARG, "no run to wait for");
+        bool retry = false;
+        for (auto &d : c->blocks) {
+            if (d.h_ctrs[C_ERR] & 4u)
+                return fail(c, TESS2_E_CAPACITY,
+                            "block %d: a cell exceeds the per-cell capacities", d.gid);
+            if (d.h_ctrs[C_ERR] & 1u) {
+                const unsigned want = std::max(2 * d.slow_cap, d.h_ctrs[C_SLOW] + 1024u);
+                int r = dalloc(c, &d.slow, want);
+                if (r) return r;
+                d.slow_cap = want;
+                retry = true;
+            }
+            if (d.h_ctrs[C_OVF] > d.ovf_cap) {
+                const unsigned want = std::max(2 * d.ovf_cap, d.h_ctrs[C_OVF] + d.h_ctrs[C_OVF] / 4);
+                int r = dalloc(c, &d.ovf, want);
+                if (r) return r;
+                d.ovf_cap = want;
+                retry = true;
+            }
+        }
+        if (!retry) break;
+        if (attempt >= 4) return fail(c, TESS2_E_CAPACITY, "list capacities did not converge");
+        int r = enqueue_run(c, c->last);
+        if (r) return r;
+    }
+    float ms = 0.0f, kms = 0.0f;
+    HIPCHK(c, hipEventElapsedTime(&ms, c->ev_run0, c->ev_run1));
+    HIPCHK(c, hipEventElapsedTime(&kms, c->ev_k0, c->ev_k1));
+    c->last_run_ms = ms;
+    c->ktime_sum += kms;
+    c->ktime_n += 1;
+    if (st) {
+        memset(st, 0, sizeof *st);
+        for (auto &d : c->blocks) {
+            st->cells_scan += (long long)d.h_cstat[0];
+            st->cells_cic += (long long)d.h_cstat[1];
+            st->cells_incomplete += (long long)d.h_cstat[2];
+            st->cells_outside += (long long)d.h_cstat[3];
+            st->cells_degenerate += (long long)d.h_cstat[4];
+            st->points_offblock += d.h_ctrs[C_OFFBLOCK];
+            st->points_offgrid += d.h_ctrs[C_OFFGRID];
+            st->points_deposited += (long long)d.h_cstat[5] - d.h_ctrs[C_OFFBLOCK] -
+                                    d.h_ctrs[C_OFFGRID];
+            st->points_shared += d.h_ctrs[C_OVF];
+            st->fp64_fallbacks += d.h_ctrs[C_FALLBACK];
+            st->tot_mass += d.h_gsum * (double)d.gp.div;
+            float mx;
+            memcpy(&mx, &d.h_gmax, sizeof mx);
+            st->max_dense = std::max(st->max_dense, mx);
+        }
+        st->device_ms = ms;
+    }
+    return 0;
+}
+
+extern "C" int tess2_kernel_time(tess2_ctx *c, float *avg_ms, int *launches)
+{
+    if (!c) return TESS2_E_ARG;
+    if (avg_ms) *avg_ms = c->ktime_n ? (float)(c->ktime_sum / c->ktime_n) : 0.0f;
+    if (launches) *launches = c->ktime_n;
+    c->ktime_sum = 0.0;
+    c->ktime_n = 0;
+    return 0;
+}
+
+extern "C" int tess2_fetch(tess2_ctx *c, int b, float *density, float *volumes,
+                           float *circumcenters)
+{
+    if (!c || b < 0 || b >= (int)c->blocks.size()) return fail(c, TESS2_E_ARG, "bad block");
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    DevBlock &d = c->blocks[b];
+    if (density && d.density)
+        HIPCHK(c, hipMemcpy(density, d.density, sizeof(float) * d.ngrid, hipMemcpyDeviceToHost));
+    if (volumes && d.norig)
+        HIPCHK(c, hipMemcpy(volumes, d.vol, sizeof(float) * d.norig, hipMemcpyDeviceToHost));
+    if (circumcenters && d.nt) {
+        std::vector<float4> tmp(d.nt);
+        HIPCHK(c, hipMemcpy(tmp.data(), d.cc, sizeof(float4) * d.nt, hipMemcpyDeviceToHost));
+        for (int t = 0; t < d.nt; ++t) {
+            circumcenters[3 * t] = tmp[t].x;
+            circumcenters[3 * t + 1] = tmp[t].y;
+            circumcenters[3 * t + 2] = tmp[t].z;
+        }
+    }
+    return 0;
+}
+
+extern "C" int tess2_dense_cells(tess2_ctx *c, int b, int *status, int *npts)
+{
+    if (!c || b < 0 || b >= (int)c->blocks.size()) return fail(c, TESS2_E_ARG, "bad block");
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    DevBlock &d = c->blocks[b];
+    if (status && d.norig)
+        HIPCHK(c, hipMemcpy(status, d.status, sizeof(int) * d.norig, hipMemcpyDeviceToHost));
+    if (npts && d.norig)
+        HIPCHK(c, hipMemcpy(npts, d.npts, sizeof(int) * d.norig, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// one-call entry points
+// ---------------------------------------------------------------------------
+extern "C" int tess2_fill_circumcenters(tess2_ctx *c, const tess2_tet_t *tets, int num_tets,
+                                        const float *particles, int num_particles, float *out)
+{
+    if (!c || num_tets < 0 || num_particles < 0 || (num_tets && (!tets || !out)))
+        return fail(c, TESS2_E_ARG, "bad arguments");
+    std::vector<int> v2t(num_particles, -1);
+    tess2_block_t b{};
+    b.num_orig_particles = 0;
+    b.num_particles = num_particles;
+    b.particles = particles;
+    b.num_tets = num_tets;
+    b.tets = tets;
+    b.vert_to_tet = v2t.data();
+    int r = tess2_stage(c, &b, 1);
+    if (r) return r;
+    DevBlock &d = c->blocks[0];
+    launch_circumcenters(d.pts, d.tets, d.nt, d.cc, c->stream);
+    HIPCHK(c, hipGetLastError());
+    return tess2_fetch(c, 0, nullptr, nullptr, out);
+}
+
+extern "C" int tess2_volumes(tess2_ctx *c, const tess2_block_t *block, float *out)
+{
+    if (!c || !block || (block->num_orig_particles && !out))
+        return fail(c, TESS2_E_ARG, "bad arguments");
+    int r = tess2_stage(c, block, 1);
+    if (r) return r;
+    DevBlock &d = c->blocks[0];
+    hipStream_t s = c->stream;
+    launch_fill_u32(d.ctrs, C_NCTR, 0u, s);
+    launch_circumcenters(d.pts, d.tets, d.nt, d.cc, s);
+    BlockIn in = make_in(d);
+    CellOut o = make_out(d, true);
+    GridP gp{};
+    launch_cells_ref(in, gp, o, 0, s);
+    launch_cells_ref_slow(in, gp, o, 0, s);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipMemcpyAsync(d.h_ctrs, d.ctrs, sizeof(unsigned) * C_NCTR, hipMemcpyDeviceToHost,
+                             s));
+    HIPCHK(c, hipStreamSynchronize(s));
+    if (d.h_ctrs[C_ERR] & 1u) {
+        // more deferred cells than the list holds: one cell per launch slot
+        int rr = dalloc(c, &d.slow, (size_t)d.norig);
+        if (rr) return rr;
+        d.slow_cap = (unsigned)std::max(d.norig, 1);
+        o = make_out(d, true);
+        launch_fill_u32(d.ctrs, C_NCTR, 0u, s);
+        launch_cells_ref(in, gp, o, 0, s);
+        launch_cells_ref_slow(in, gp, o, 0, s);
+        HIPCHK(c, hipMemcpyAsync(d.h_ctrs, d.ctrs, sizeof(unsigned) * C_NCTR,
+                                 hipMemcpyDeviceToHost, s));
+        HIPCHK(c, hipStreamSynchronize(s));
+    }
+    if (d.h_ctrs[C_ERR] & 4u)
+        return fail(c, TESS2_E_CAPACITY, "a cell exceeds the per-cell capacities");
+    return tess2_fetch(c, 0, nullptr, out, nullptr);
+}
+
+extern "C" int tess2_dense(tess2_ctx *c, int alg, int num_given_bounds, const float *given_mins,
+                           const float *given_maxs, int project, const float *proj_plane,
+                           float mass, float *data_mins, float *data_maxs, float *grid_phys_mins,
+                           float *grid_phys_maxs, float *grid_step_size, float eps,
+                           const int *glo_num_idx, const tess2_block_t *blocks, int nblocks,
+                           int assignment, int precision, float **density_out,
+                           tess2_stats_t *stats)
+{
+    if (!c || !blocks || nblocks <= 0 || !density_out) return fail(c, TESS2_E_ARG, "bad args");
+    int r = tess2_stage(c, blocks, nblocks);
+    if (r) return r;
+    r = tess2_run(c, alg, num_given_bounds, given_mins, given_maxs, project, proj_plane, mass,
+                  eps, glo_num_idx, assignment, precision, 0);
+    if (r) return r;
+    r = tess2_sync(c, stats);
+    if (r) return r;
+    for (int i = 0; i < 3; ++i) {
+        if (data_mins) data_mins[i] = c->data_mins[i];
+        if (data_maxs) data_maxs[i] = c->data_maxs[i];
+        if (grid_phys_mins) grid_phys_mins[i] = c->gmin[i];
+        if (grid_phys_maxs) grid_phys_maxs[i] = c->gmax[i];
+        if (grid_step_size) grid_step_size[i] = c->step[i];
+    }
+    for (int k = 0; k < nblocks; ++k) {
+        if (!density_out[k]) continue;
+        r = tess2_fetch(c, k, density_out[k], nullptr, nullptr);
+        if (r) return r;
+    }
+    return 0;
+}
+
+extern "C" int tess2_write_grid(const char *path, const float *density, const int *gnum,
+                                int project)
+{
+    if (!path || !density || !gnum) return TESS2_E_ARG;
+    FILE *f = fopen(path, "wb");   // truncates, as MPI_File_set_size(fd, 0)
+    if (!f) return TESS2_E_IO;
+    const size_t n = (size_t)gnum[0] * gnum[1] * (project ? 1 : (size_t)gnum[2]);
+    const size_t w = fwrite(density, sizeof(float), n, f);
+    const int e = fclose(f);
+    return (w == n && e == 0) ? 0 : TESS2_E_IO;
+}

@@ -1,0 +1,718 @@
+// cells_ref.hip -- gfx950 kernels of the reference-exact density path.
+//
+// One 64-lane wavefront handles one Voronoi cell (one original particle):
+//   1. star: level-synchronous BFS over the tets around the site, starting at
+//      vert_to_tet[v], that visits tets in exactly the first-seen order of
+//      neighbor_edges() (src/tet.cpp:228-270); each BFS level is one round of
+//      parallel gathers, duplicates are resolved with wave ballots.  Star tet
+//      records and their circumcenters are staged in LDS.
+//   2. faces: the Delaunay neighbours u in first-seen order with their first
+//      tet ut (one face per lane); each lane walks its edge ring in LDS
+//      (fill_edge_link, src/tet.cpp:389-409) and accumulates, in ring order,
+//      the Newell normal (src/dense.cpp:1131-1162), the fan area
+//      (src/volume.cpp:32-46) and the ring's bounding box.
+//   3. volume = sum of area*|u-v|/6 over faces in neighbour order
+//      (src/volume.cpp:23-53).
+//   4. data-bounds filter and bbox -> grid range (src/dense.cpp:1381-1410).
+//   5. scanline search (CellInteriorGridPts, src/dense.cpp:1475-1701) as a
+//      wave-uniform state machine; every PtInCell probe evaluates all faces
+//      at once, one per lane, and two ballots give the verdict
+//      (src/dense.cpp:1172-1203: inside iff no two significant faces
+//      disagree in sign).
+//   6. deposit: mass/N per found point (or CIC of the site when N = 0).
+//      The first contribution to a grid point writes the density directly;
+//      later ones go to a shared-point list that an ordered pass folds in
+//      ascending cell id, replaying density += mass/div (float += double,
+//      src/dense.cpp:290) in the reference's order.
+#include "kernels.hpp"
+#include "geom.hpp"
+
+namespace t2 {
+
+enum { ST_SCAN = 0, ST_CIC = 1, ST_INCOMPLETE = 2, ST_OUTSIDE = 3, ST_DEGENERATE = 4 };
+
+__device__ __forceinline__ int lane_id() { return (int)(threadIdx.x & 63); }
+
+__device__ __forceinline__ int rdl(int x, int l) { return __builtin_amdgcn_readlane(x, l); }
+__device__ __forceinline__ float rdlf(float x, int l)
+{
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), l));
+}
+
+__device__ __forceinline__ int comp(const int4 &q, int i)
+{
+    return i == 0 ? q.x : (i == 1 ? q.y : (i == 2 ? q.z : q.w));
+}
+
+// ---------------------------------------------------------------------------
+// deposit one grid point (src/dense.cpp:276-305)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void deposit_point(int gx, int gy, int gz, float m, int cell,
+                                              const GridP &gp, const CellOut &o)
+{
+    const float px = idx2phys1(gx, gp.step[0], gp.gmin[0]);
+    const float py = idx2phys1(gy, gp.step[1], gp.gmin[1]);
+    const float pz = idx2phys1(gz, gp.step[2], gp.gmin[2]);
+    if (!(px >= gp.bmin[0] && px <= gp.bmax[0] && py >= gp.bmin[1] && py <= gp.bmax[1] &&
+          pz >= gp.bmin[2] && pz <= gp.bmax[2])) {
+        atomicAdd(&o.ctrs[C_OFFBLOCK], 1u);
+        return;
+    }
+    const int lx = gx - gp.bmin_idx[0], ly = gy - gp.bmin_idx[1], lz = gz - gp.bmin_idx[2];
+    if (lx < 0 || ly < 0 || lx >= gp.bnum_idx[0] || ly >= gp.bnum_idx[1] ||
+        (!gp.project && (lz < 0 || lz >= gp.bnum_idx[2]))) {
+        atomicAdd(&o.ctrs[C_OFFGRID], 1u);
+        return;
+    }
+    const long long lidx =
+        gp.project ? (long long)ly * gp.bnum_idx[0] + lx
+                   : ((long long)lz * gp.bnum_idx[1] + ly) * gp.bnum_idx[0] + lx;
+    const unsigned c = atomicAdd(&o.count[lidx], 1u);
+    if (c == 0u) {
+        o.first[lidx] = make_uint2((unsigned)cell, __float_as_uint(m));
+        o.density[lidx] = (float)((double)m / (double)gp.div);
+    } else {
+        const unsigned e = atomicAdd(&o.ctrs[C_OVF], 1u);
+        if (e < o.ovf_cap) {
+            const int prev = atomicExch(&o.head[lidx], (int)e);
+            o.ovf[e] = make_int4((int)lidx, cell, __float_as_int(m), prev);
+        }
+    }
+}
+
+// DistributeScalarCIC(): src/dense.cpp:1787-1872, corner k in z,y,x order.
+__device__ void cic_weights(float sx, float sy, float sz, float scalar, const GridP &gp,
+                            int mn[3], float m8[8])
+{
+    mn[0] = phys2idx1(sx, gp.step[0], gp.gmin[0]);
+    mn[1] = phys2idx1(sy, gp.step[1], gp.gmin[1]);
+    mn[2] = phys2idx1(sz, gp.step[2], gp.gmin[2]);
+    float tot = 0.0f, v0 = 0.0f;
+    const float e2 = 2.0f * gp.eps;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const int ix = mn[0] + (k & 1), iy = mn[1] + ((k >> 1) & 1), iz = mn[2] + (k >> 2);
+        const float gx = idx2phys1(ix, gp.step[0], gp.gmin[0]);
+        const float gy = idx2phys1(iy, gp.step[1], gp.gmin[1]);
+        const float gz = idx2phys1(iz, gp.step[2], gp.gmin[2]);
+        float px = sx, py = sy, pz = sz;
+        if (fabsf(px - gx) < gp.eps) px += e2;
+        if (fabsf(py - gy) < gp.eps) py += e2;
+        if (fabsf(pz - gz) < gp.eps) pz += e2;
+        const float vol = fabsf((gx - px) * (gy - py) * (gz - pz));
+        if (v0 == 0.0f) v0 = vol;
+        const float w = v0 / vol;
+        m8[k] = w;
+        tot += w;
+    }
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const float w = m8[k] / tot;
+        m8[k] = w * scalar;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// the per-cell wavefront kernel
+//   NS   : star capacity in units of 64 tets
+//   NF   : face capacity in units of 64 faces
+//   RMAX : capacity of the scan's run list
+// ---------------------------------------------------------------------------
+template <int NS, int NF, int RMAX>
+__device__ void cell_ref_body(const int v, const BlockIn &in, const GridP &gp,
+                              const CellOut &o, const int want_density, const bool can_defer,
+                              int4 *s_v, int *s_ns, float4 *s_cc, int4 *s_u)
+{
+    constexpr int SMAX = 64 * NS;
+    constexpr int FMAX = 64 * NF;
+    constexpr int LMAX = 4096;
+    const int lane = lane_id();
+
+    const int vt = in.v2t[v];
+    if (vt < 0) {
+        if (lane == 0) {
+            o.status[v] = ST_INCOMPLETE;
+            o.npts[v] = 0;
+            if (o.vol) o.vol[v] = -1.0f;
+        }
+        return;
+    }
+
+    // ---- 1. star BFS -------------------------------------------------------
+    int sid[NS];
+#pragma unroll
+    for (int k = 0; k < NS; ++k) sid[k] = -1;
+    if (lane == 0) {
+        s_v[0] = in.tets[2 * vt];
+        s_u[0] = in.tets[2 * vt + 1];
+        s_cc[0] = in.cc[vt];
+        sid[0] = vt;
+    }
+    int S = 1, f0 = 0, f1 = 1;
+    bool hull = false, over = false;
+    __syncthreads();
+    while (f0 < f1) {
+        const int ncand = (f1 - f0) * 4;
+        for (int base = 0; base < ncand; base += 64) {
+            const int key = base + lane;
+            int cand = -1, s = 0, i = 0;
+            bool valid = false, h = false;
+            if (key < ncand) {
+                s = f0 + (key >> 2);
+                i = key & 3;
+                const int vi = comp(s_v[s], i);
+                const int ni = comp(s_u[s], i);
+                if (vi != v) {
+                    if (ni < 0) h = true;
+                    else {
+                        valid = true;
+                        cand = ni;
+                    }
+                }
+            }
+            if (__ballot(h)) {
+                hull = true;
+                break;
+            }
+            unsigned long long pend = __ballot(valid);
+            int myslot = -1;
+            while (pend) {
+                const int leader = __ffsll((unsigned long long)pend) - 1;
+                const int val = rdl(cand, leader);
+                const unsigned long long same = __ballot(valid && cand == val);
+                int slot = -1;
+#pragma unroll
+                for (int k = 0; k < NS; ++k) {
+                    const unsigned long long m = __ballot(sid[k] == val);
+                    if (slot < 0 && m) slot = k * 64 + __ffsll(m) - 1;
+                }
+                if (slot < 0) {
+                    slot = S;
+                    if (S < SMAX) {
+#pragma unroll
+                        for (int k = 0; k < NS; ++k)
+                            if (k == (S >> 6) && lane == (S & 63)) sid[k] = val;
+                    }
+                    S++;
+                }
+                if ((same >> lane) & 1ull) myslot = slot;
+                pend &= ~same;
+            }
+            if (key < ncand) s_ns[s * 4 + i] = valid ? myslot : -1;
+        }
+        if (hull) break;
+        if (S > SMAX) {
+            over = true;
+            break;
+        }
+        // gather the records of the new slots [f1, S)
+        for (int b = f1; b < S; b += 64) {
+            const int s = b + lane;
+            int id = -1;
+#pragma unroll
+            for (int k = 0; k < NS; ++k) {
+                const int t = __shfl(sid[k], s & 63);
+                if ((s >> 6) == k) id = t;
+            }
+            if (s < S) {
+                s_v[s] = in.tets[2 * id];
+                s_u[s] = in.tets[2 * id + 1];
+                s_cc[s] = in.cc[id];
+            }
+        }
+        __syncthreads();
+        f0 = f1;
+        f1 = S;
+    }
+    if (hull) {
+        if (lane == 0) {
+            o.status[v] = ST_INCOMPLETE;
+            o.npts[v] = 0;
+            if (o.vol) o.vol[v] = -1.0f;
+        }
+        return;
+    }
+    if (over) goto defer;
+
+    {
+        // ---- 2. faces: neighbours in first-seen order -----------------------
+        int fu[NF], fs[NF];
+#pragma unroll
+        for (int j = 0; j < NF; ++j) {
+            fu[j] = -1;
+            fs[j] = 0;
+        }
+        int F = 0;
+        const int nent = S * 4;
+        for (int base = 0; base < nent; base += 64) {
+            const int e = base + lane;
+            int u = -1;
+            bool valid = false;
+            if (e < nent) {
+                u = comp(s_v[e >> 2], e & 3);
+                valid = (u != v);
+            }
+            unsigned long long pend = __ballot(valid);
+            while (pend) {
+                const int leader = __ffsll((unsigned long long)pend) - 1;
+                const int val = rdl(u, leader);
+                const unsigned long long same = __ballot(valid && u == val);
+                bool found = false;
+#pragma unroll
+                for (int j = 0; j < NF; ++j)
+                    if (__ballot(fu[j] == val)) found = true;
+                if (!found) {
+                    if (F < FMAX) {
+#pragma unroll
+                        for (int j = 0; j < NF; ++j)
+                            if (j == (F >> 6) && lane == (F & 63)) {
+                                fu[j] = val;
+                                fs[j] = (base + leader) >> 2;
+                            }
+                    }
+                    F++;
+                }
+                pend &= ~same;
+            }
+        }
+        if (F > FMAX) goto defer;
+
+        // ---- ring walks: Newell normal, fan area, bbox (one face per lane) --
+        const float4 site = in.pts[v];
+        float fnx[NF], fny[NF], fnz[NF], ffx[NF], ffy[NF], ffz[NF], fvol[NF];
+        float mnx = INFINITY, mny = INFINITY, mnz = INFINITY;
+        float mxx = -INFINITY, mxy = -INFINITY, mxz = -INFINITY;
+        bool ring_bad = false;
+#pragma unroll
+        for (int j = 0; j < NF; ++j) {
+            fnx[j] = fny[j] = fnz[j] = 0.0f;
+            ffx[j] = ffy[j] = ffz[j] = 0.0f;
+            fvol[j] = 0.0f;
+            if (j * 64 + lane < F) {
+                const int u = fu[j];
+                const int us = fs[j];
+                int4 tv = s_v[us];
+                int wi = 0;
+                for (int i = 3; i >= 0; --i) {   // circulate_start: first slot not v,u
+                    const int w = comp(tv, i);
+                    if (w != v && w != u) wi = i;
+                }
+                const float4 c0 = s_cc[us];
+                float cx = c0.x, cy = c0.y, cz = c0.z;
+                float nx = 0.0f, ny = 0.0f, nz = 0.0f;
+                float area = 0.0f;
+                if (c0.x < mnx) mnx = c0.x;
+                if (c0.y < mny) mny = c0.y;
+                if (c0.z < mnz) mnz = c0.z;
+                if (c0.x > mxx) mxx = c0.x;
+                if (c0.y > mxy) mxy = c0.y;
+                if (c0.z > mxz) mxz = c0.z;
+                int t = us, L = 1;
+                for (;;) {
+                    // circulate_next (src/tet.cpp:185-215)
+                    int nvid = -1;
+                    for (int i = 3; i >= 0; --i) {
+                        const int w = comp(tv, i);
+                        if (i != wi && w != v && w != u) nvid = w;
+                    }
+                    const int nt = s_ns[t * 4 + wi];
+                    if (nt == us || nt < 0) break;
+                    const int4 nv4 = s_v[nt];
+                    int nwi = -1;
+                    for (int i = 3; i >= 0; --i)
+                        if (comp(nv4, i) == nvid) nwi = i;
+                    const float4 c = s_cc[nt];
+                    // Newell term for (ring[L-1], ring[L])
+                    nx += (cy - c.y) * (cz + c.z);
+                    ny += (cz - c.z) * (cx + c.x);
+                    nz += (cx - c.x) * (cy + c.y);
+                    if (L >= 2) {   // fan triangle (ring[0], ring[L-1], ring[L])
+                        const float abx = cx - c0.x, aby = cy - c0.y, abz = cz - c0.z;
+                        const float acx = c.x - c0.x, acy = c.y - c0.y, acz = c.z - c0.z;
+                        const float px = aby * acz - abz * acy;
+                        const float py = abz * acx - abx * acz;
+                        const float pz = abx * acy - aby * acx;
+                        area = (float)((double)area + sqrt((double)nrm3(px, py, pz)) / 2.0);
+                    }
+                    if (c.x < mnx) mnx = c.x;
+                    if (c.y < mny) mny = c.y;
+                    if (c.z < mnz) mnz = c.z;
+                    if (c.x > mxx) mxx = c.x;
+                    if (c.y > mxy) mxy = c.y;
+                    if (c.z > mxz) mxz = c.z;
+                    cx = c.x;
+                    cy = c.y;
+                    cz = c.z;
+                    t = nt;
+                    tv = nv4;
+                    wi = nwi;
+                    L++;
+                    if (nwi < 0 || L > LMAX) {
+                        ring_bad = (L > LMAX);
+                        break;
+                    }
+                }
+                // closing Newell term (ring[L-1], ring[0])
+                nx += (cy - c0.y) * (cz + c0.z);
+                ny += (cz - c0.z) * (cx + c0.x);
+                nz += (cx - c0.x) * (cy + c0.y);
+                const float mag = sqrtf(nx * nx + ny * ny + nz * nz);
+                nx = -(nx / mag);
+                ny = -(ny / mag);
+                nz = -(nz / mag);
+                const float wx = c0.x - site.x, wy = c0.y - site.y, wz = c0.z - site.z;
+                if (wx * nx + wy * ny + wz * nz < 0.0f) {
+                    nx = -nx;
+                    ny = -ny;
+                    nz = -nz;
+                }
+                fnx[j] = nx;
+                fny[j] = ny;
+                fnz[j] = nz;
+                ffx[j] = c0.x;
+                ffy[j] = c0.y;
+                ffz[j] = c0.z;
+                const float4 pu = in.pts[u];
+                const float ddx = pu.x - site.x, ddy = pu.y - site.y, ddz = pu.z - site.z;
+                const float dist = sqrtf(nrm3(ddx, ddy, ddz));
+                fvol[j] = area * dist / 6.0f;
+            }
+        }
+        if (__ballot(ring_bad)) goto defer;
+
+        // ---- 3. volume: faces summed in neighbour order ----------------------
+        if (o.vol) {
+            float vol = 0.0f;
+            for (int k = 0; k < F; ++k) {
+                float x = 0.0f;
+#pragma unroll
+                for (int j = 0; j < NF; ++j)
+                    if ((k >> 6) == j) x = rdlf(fvol[j], k & 63);
+                vol += x;
+            }
+            if (lane == 0) o.vol[v] = vol;
+        }
+        if (!want_density) {
+            if (lane == 0) {
+                o.status[v] = ST_SCAN;
+                o.npts[v] = 0;
+            }
+            return;
+        }
+
+        // ---- 4. cell bbox, data-bounds filter (src/dense.cpp:701-714,1381-1392)
+        for (int off = 32; off >= 1; off >>= 1) {
+            mnx = fminf(mnx, __shfl_xor(mnx, off));
+            mny = fminf(mny, __shfl_xor(mny, off));
+            mnz = fminf(mnz, __shfl_xor(mnz, off));
+            mxx = fmaxf(mxx, __shfl_xor(mxx, off));
+            mxy = fmaxf(mxy, __shfl_xor(mxy, off));
+            mxz = fmaxf(mxz, __shfl_xor(mxz, off));
+        }
+        {
+            // the first ring vertex of the first face seeds the reference's
+            // fold; a NaN there survives every later comparison
+            const float4 c00 = s_cc[rdl(fs[0], 0)];
+            if (isnan(c00.x)) { mnx = c00.x; mxx = c00.x; }
+            if (isnan(c00.y)) { mny = c00.y; mxy = c00.y; }
+            if (isnan(c00.z)) { mnz = c00.z; mxz = c00.z; }
+        }
+        const float ex = (gp.dmax[0] - gp.dmin[0]) * 2.0f * FLT_EPSILON;
+        const float ey = (gp.dmax[1] - gp.dmin[1]) * 2.0f * FLT_EPSILON;
+        const float ez = (gp.dmax[2] - gp.dmin[2]) * 2.0f * FLT_EPSILON;
+        if (mnx < gp.dmin[0] - ex || mny < gp.dmin[1] - ey || mnz < gp.dmin[2] - ez ||
+            mxx > gp.dmax[0] + ex || mxy > gp.dmax[1] + ey || mxz > gp.dmax[2] + ez) {
+            if (lane == 0) {
+                o.status[v] = ST_OUTSIDE;
+                o.npts[v] = 0;
+            }
+            return;
+        }
+        if (isnan(mnx) || isnan(mny) || isnan(mnz) || isnan(mxx) || isnan(mxy) || isnan(mxz)) {
+            if (lane == 0) {
+                o.status[v] = ST_DEGENERATE;
+                o.npts[v] = 0;
+            }
+            return;
+        }
+        const int cmi0 = phys2idx1(mnx, gp.step[0], gp.gmin[0]);
+        const int cmi1 = phys2idx1(mny, gp.step[1], gp.gmin[1]);
+        const int cmi2 = phys2idx1(mnz, gp.step[2], gp.gmin[2]);
+        const int n0 = phys2idx1(mxx, gp.step[0], gp.gmin[0]) - cmi0 + 1;
+        const int n1 = phys2idx1(mxy, gp.step[1], gp.gmin[1]) - cmi1 + 1;
+        const int n2 = phys2idx1(mxz, gp.step[2], gp.gmin[2]) - cmi2 + 1;
+        const float p0 = idx2phys1(cmi0, gp.step[0], gp.gmin[0]);
+        const float p1 = idx2phys1(cmi1, gp.step[1], gp.gmin[1]);
+        const float p2 = idx2phys1(cmi2, gp.step[2], gp.gmin[2]);
+        const float eps = gp.eps;
+
+        // PtInCell over all faces at once
+        auto inside = [&](float qx, float qy, float qz) -> bool {
+            bool pos = false, neg = false;
+#pragma unroll
+            for (int j = 0; j < NF; ++j) {
+                if (j * 64 + lane < F) {
+                    const float d = fnx[j] * (qx - ffx[j]) + fny[j] * (qy - ffy[j]) +
+                                    fnz[j] * (qz - ffz[j]);
+                    if (fabsf(d) > eps) {
+                        if (d >= 0.0f) pos = true;
+                        else neg = true;
+                    }
+                }
+            }
+            return !(__ballot(pos) != 0ull && __ballot(neg) != 0ull);
+        };
+
+        // ---- 5. scanline search (src/dense.cpp:1486-1676) --------------------
+        int4 *runs = s_u;   // star neighbour records are no longer needed
+        int nruns = 0, tot = 0;
+        int x_left = n0 / 2, x_right = n0 / 2, y_start = 0, first_x = 0;
+        bool run_over = false;
+        for (int zi = 0; zi < n2 && !run_over; zi++) {
+            bool border_found = false, z_step_done = false;
+            const float gz = p2 + (float)zi * gp.step[2];
+            for (int yi = y_start; yi < n1; yi++) {
+                const float gy = p1 + (float)yi * gp.step[1];
+                const bool x_in = inside(p0 + (float)x_left * gp.step[0], gy, gz);
+                int min_xi = n0 - 1, max_xi = 0;
+                for (int xi = x_left; xi >= 0 && xi < n0;) {
+                    if (inside(p0 + (float)xi * gp.step[0], gy, gz)) {
+                        if (xi < min_xi) min_xi = xi;
+                        if (xi > max_xi) max_xi = xi;
+                        if (x_in) xi--;
+                        else {
+                            x_left = xi;
+                            break;
+                        }
+                    } else {
+                        if (!x_in) xi++;
+                        else {
+                            x_left = xi;
+                            break;
+                        }
+                    }
+                }
+                for (int xi = x_right; xi >= 0 && xi < n0;) {
+                    if (inside(p0 + (float)xi * gp.step[0], gy, gz)) {
+                        if (xi < min_xi) min_xi = xi;
+                        if (xi > max_xi) max_xi = xi;
+                        if (x_in) xi++;
+                        else {
+                            x_right = xi;
+                            break;
+                        }
+                    } else {
+                        if (!x_in) xi--;
+                        else {
+                            x_right = xi;
+                            break;
+                        }
+                    }
+                }
+                int first_y = y_start;
+                if (min_xi <= max_xi) {
+                    if (nruns >= RMAX) {
+                        run_over = true;
+                        break;
+                    }
+                    if (lane == 0) runs[nruns] = make_int4(zi, yi, min_xi, tot);
+                    nruns++;
+                    tot += (max_xi - min_xi + 1);
+                    if (yi == y_start) first_x = (min_xi + max_xi) / 2;
+                    if (!border_found) {
+                        first_y = yi;
+                        border_found = true;
+                    }
+                } else if (border_found) {
+                    z_step_done = true;
+                }
+                if ((yi == n1 - 1 || z_step_done) && zi + 1 < n2) {
+                    const float gz1 = p2 + (float)(zi + 1) * gp.step[2];
+                    const float gx1 = p0 + (float)first_x * gp.step[0];
+                    int yj;
+                    for (yj = first_y; yj > 0; yj--) {
+                        if (!inside(gx1, p1 + (float)yj * gp.step[1], gz1)) break;
+                    }
+                    y_start = yj;
+                }
+                if (z_step_done) break;
+            }
+        }
+        if (run_over) goto defer;
+        __syncthreads();
+
+        // ---- 6. deposit -------------------------------------------------------
+        if (tot > 0) {
+            const float mpp = gp.mass / (float)tot;   // src/dense.cpp:1692
+            for (int base = 0; base < tot; base += 64) {
+                const int p = base + lane;
+                int r = 0;
+                for (int k = 1; k < nruns; ++k)
+                    if (p >= runs[k].w) r = k;
+                if (p < tot) {
+                    const int4 rr = runs[r];
+                    deposit_point(cmi0 + rr.z + (p - rr.w), cmi1 + rr.y, cmi2 + rr.x, mpp, v,
+                                  gp, o);
+                }
+            }
+            if (lane == 0) {
+                o.status[v] = ST_SCAN;
+                o.npts[v] = tot;
+            }
+        } else {
+            int mn[3];
+            float m8[8];
+            cic_weights(site.x, site.y, site.z, gp.mass, gp, mn, m8);
+            if (lane < 8) {
+                float m = m8[0];
+#pragma unroll
+                for (int k = 1; k < 8; ++k)
+                    if (lane == k) m = m8[k];
+                deposit_point(mn[0] + (lane & 1), mn[1] + ((lane >> 1) & 1), mn[2] + (lane >> 2),
+                              m, v, gp, o);
+            }
+            if (lane == 0) {
+                o.status[v] = ST_CIC;
+                o.npts[v] = 8;
+            }
+        }
+        return;
+    }
+
+defer:
+    if (lane == 0) {
+        if (can_defer) {
+            const unsigned k = atomicAdd(&o.ctrs[C_SLOW], 1u);
+            if (k < o.slow_cap) o.slow[k] = v;
+            else atomicOr(&o.ctrs[C_ERR], 1u);   // list full: rerun with more room
+        } else {
+            atomicOr(&o.ctrs[C_ERR], 4u);        // beyond the large-capacity pass
+            o.status[v] = -1;
+            o.npts[v] = 0;
+        }
+    }
+}
+
+template <int NS, int NF, int RMAX>
+__global__ __launch_bounds__(64) void k_cells_ref(BlockIn in, GridP gp, CellOut o,
+                                                  int want_density)
+{
+    __shared__ int4 s_v[64 * NS];
+    __shared__ int s_ns[64 * NS * 4];
+    __shared__ float4 s_cc[64 * NS];
+    __shared__ int4 s_u[(64 * NS > RMAX) ? 64 * NS : RMAX];
+    const int v = blockIdx.x;
+    if (v >= in.ncells) return;
+    cell_ref_body<NS, NF, RMAX>(v, in, gp, o, want_density, true, s_v, s_ns, s_cc, s_u);
+}
+
+// large-capacity pass over the cells the fast kernel deferred; the grid is
+// fixed and each workgroup strides over the deferred list
+template <int NS, int NF, int RMAX>
+__global__ __launch_bounds__(64) void k_cells_ref_slow(BlockIn in, GridP gp, CellOut o,
+                                                       int want_density)
+{
+    __shared__ int4 s_v[64 * NS];
+    __shared__ int s_ns[64 * NS * 4];
+    __shared__ float4 s_cc[64 * NS];
+    __shared__ int4 s_u[(64 * NS > RMAX) ? 64 * NS : RMAX];
+    unsigned n = o.ctrs[C_SLOW];
+    if (n > o.slow_cap) n = o.slow_cap;
+    for (unsigned k = blockIdx.x; k < n; k += gridDim.x) {
+        const int v = o.slow[k];
+        cell_ref_body<NS, NF, RMAX>(v, in, gp, o, want_density, false, s_v, s_ns, s_cc, s_u);
+        __syncthreads();
+    }
+}
+
+void launch_cells_ref(const BlockIn &in, const GridP &gp, const CellOut &o, int want_density,
+                      hipStream_t s)
+{
+    if (in.ncells <= 0) return;
+    hipLaunchKernelGGL((k_cells_ref<2, 1, 128>), dim3(in.ncells), dim3(64), 0, s, in, gp, o,
+                       want_density);
+}
+
+void launch_cells_ref_slow(const BlockIn &in, const GridP &gp, const CellOut &o,
+                           int want_density, hipStream_t s)
+{
+    if (in.ncells <= 0) return;
+    hipLaunchKernelGGL((k_cells_ref_slow<8, 2, 2048>), dim3(512), dim3(64), 0, s, in, gp,
+                       o, want_density);
+}
+
+// ---------------------------------------------------------------------------
+// DENSE_CIC estimator: IterateCellsCic, src/dense.cpp:486-562
+// ---------------------------------------------------------------------------
+__global__ void k_cic(BlockIn in, GridP gp, CellOut o)
+{
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= in.ncells) return;
+    const float4 p = in.pts[v];
+    int mn[3];
+    float m8[8];
+    cic_weights(p.x, p.y, p.z, gp.mass, gp, mn, m8);
+    for (int k = 0; k < 8; ++k)
+        deposit_point(mn[0] + (k & 1), mn[1] + ((k >> 1) & 1), mn[2] + (k >> 2), m8[k], v,
+                      gp, o);
+    o.status[v] = ST_CIC;
+    o.npts[v] = 8;
+}
+
+void launch_cic(const BlockIn &in, const GridP &gp, const CellOut &o, hipStream_t s)
+{
+    if (in.ncells <= 0) return;
+    hipLaunchKernelGGL(k_cic, dim3((in.ncells + 255) / 256), dim3(256), 0, s, in, gp, o);
+}
+
+// ---------------------------------------------------------------------------
+// ordered fold of shared grid points: contributions in ascending cell id,
+// density = (float)((double)density + mass/div) each (src/dense.cpp:290)
+// ---------------------------------------------------------------------------
+__global__ void k_finalize(CellOut o, float div)
+{
+    const unsigned n = o.ctrs[C_OVF];
+    if (n > o.ovf_cap) return;   // list overflowed: the host reruns with more room
+    for (unsigned e = blockIdx.x * blockDim.x + threadIdx.x; e < n;
+         e += gridDim.x * blockDim.x) {
+        const int4 r = o.ovf[e];
+        const int lidx = r.x;
+        if (o.head[lidx] != (int)e) continue;
+        const unsigned cnt = o.count[lidx];
+        const uint2 fst = o.first[lidx];
+        float rho = 0.0f;
+        int prev = -1;
+        for (unsigned k = 0; k < cnt; ++k) {
+            // next contribution: smallest cell id above prev
+            int best = 0x7fffffff;
+            float bm = 0.0f;
+            if ((int)fst.x > prev && (int)fst.x < best) {
+                best = (int)fst.x;
+                bm = __uint_as_float(fst.y);
+            }
+            int q = (int)e;
+            for (unsigned j = 0; j + 1 < cnt; ++j) {
+                const int4 rq = o.ovf[q];
+                if (rq.y > prev && rq.y < best) {
+                    best = rq.y;
+                    bm = __int_as_float(rq.z);
+                }
+                q = rq.w;
+            }
+            rho = (float)((double)rho + (double)bm / (double)div);
+            prev = best;
+        }
+        o.density[lidx] = rho;
+    }
+}
+
+void launch_finalize(const CellOut &o, const GridP &gp, hipStream_t s)
+{
+    const unsigned cap = o.ovf_cap;
+    unsigned blocks = (cap + 255) / 256;
+    if (blocks > 4096) blocks = 4096;
+    if (blocks == 0) blocks = 1;
+    hipLaunchKernelGGL(k_finalize, dim3(blocks), dim3(256), 0, s, o, gp.div);
+}
+
+}  // namespace t2

@@ -1,0 +1,76 @@
+// kernels.hpp -- device data layout and kernel declarations.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace t2 {
+
+// Grid and block parameters of one dense run on one block (host-computed
+// with the reference's GridStepParams / BlockGridParams arithmetic).
+struct GridP {
+    float gmin[3];
+    float step[3];
+    float dmin[3];
+    float dmax[3];
+    float bmin[3];       // block bounds (the in-block deposit test)
+    float bmax[3];
+    int bmin_idx[3];
+    int bnum_idx[3];
+    float eps;
+    float mass;
+    float div;
+    int project;         // 2-D x-y projection
+};
+
+// counters in device memory (ctrs[])
+enum {
+    C_OVF = 0,        // entries appended to the shared-point list
+    C_OFFBLOCK = 1,   // points outside the block bounds (dropped)
+    C_SLOW = 2,       // cells queued for the large-capacity pass
+    C_ERR = 3,        // bit 0: deferred-cell list full; bit 2: cell beyond all capacities
+    C_OFFGRID = 4,    // in-bounds points whose index left the block grid
+    C_FALLBACK = 5,   // FP64 re-checks of the exact assignment
+    C_NCTR = 8
+};
+
+// Per-block device outputs of the density kernels.
+struct CellOut {
+    float *vol;        // [ncells] volume() or nullptr
+    int *status;       // [ncells] TESS2_CELL_*
+    int *npts;         // [ncells] points the cell deposits
+    float *density;    // [grid]
+    unsigned *count;   // [grid] contributions per point
+    uint2 *first;      // [grid] first contribution (cell, mass bits)
+    int *head;         // [grid] newest shared-list entry of the point
+    int4 *ovf;         // shared-point list: {lidx, cell, mass bits, next}
+    unsigned ovf_cap;
+    unsigned *ctrs;    // [C_NCTR]
+    int *slow;         // cells for the large-capacity pass
+    unsigned slow_cap;
+};
+
+struct BlockIn {
+    const float4 *pts;   // xyz + pad
+    const int4 *tets;    // 2 int4 per tet: verts, neighbours
+    const int *v2t;
+    const float4 *cc;    // circumcenters (xyz + pad)
+    int ncells;          // original particles
+    int ntets;
+};
+
+void launch_circumcenters(const float4 *pts, const int4 *tets, int ntets, float4 *cc,
+                          hipStream_t s);
+void launch_fill_u32(unsigned *p, size_t n, unsigned v, hipStream_t s);
+void launch_cells_ref(const BlockIn &in, const GridP &gp, const CellOut &o,
+                      int want_density, hipStream_t s);
+void launch_cells_ref_slow(const BlockIn &in, const GridP &gp, const CellOut &o,
+                           int want_density, hipStream_t s);
+void launch_cic(const BlockIn &in, const GridP &gp, const CellOut &o, hipStream_t s);
+void launch_finalize(const CellOut &o, const GridP &gp, hipStream_t s);
+void launch_cell_stats(const int *status, const int *npts, int ncells,
+                       unsigned long long *out6, hipStream_t s);
+void launch_grid_stats(const float *density, size_t n, double *sum, unsigned *maxbits,
+                       hipStream_t s);
+
+}  // namespace t2

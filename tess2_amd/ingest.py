@@ -1,0 +1,283 @@
+"""Host-side ingestion: particles -> Qhull Delaunay -> tet_t arrays.
+
+The reference computes each block's Delaunay tessellation with Qhull
+("qhull d Qt", src/tess-qhull.c:24-73), converts facets to tet_t with
+tets[i] opposite verts[i] (src/tess-qhull.c:83-225) and picks, for every
+vertex, the last tet that contains it (fill_vert_to_tet, src/tess.cpp:767-787).
+
+Here Qhull is SciPy's bundled copy, called as Delaunay(points, "Qt"): SciPy
+prefixes "d" and documents neighbors[j] as opposite simplices[j], the tet_t
+convention.  Periodic images ("ghosts") are appended after the original
+particles, as the reference's ghost exchange appends received particles
+(include/tess/delaunay.h:44-49, wrap_pt src/tess.cpp:703-710).
+
+For large inputs the tessellation is computed per x-y column in worker
+processes, each column with a margin of neighbouring particles; a tet is
+kept by the column holding its lowest-id vertex, only if its circumsphere
+(clipped to the cloud's bounding box) lies inside that column's padded
+region, so that it is Delaunay for the whole set; neighbour links are then
+rebuilt across columns by matching shared faces.  Generic (non-cospherical)
+inputs have a unique Delaunay tessellation, so the merge equals a single
+Qhull call up to tet numbering.
+"""
+from __future__ import annotations
+
+import os
+import sys
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+
+
+@dataclass
+class HostBlock:
+    """One block: the dblock_t / DBlock fields the density path reads."""
+    particles: np.ndarray          # (num_particles, 3) float32, originals first
+    tets: np.ndarray               # (num_tets, 8) int32: verts[4], tets[4]
+    vert_to_tet: np.ndarray        # (num_particles,) int32
+    num_orig: int
+    bounds_min: np.ndarray         # (3,) float32
+    bounds_max: np.ndarray         # (3,) float32
+    gid: int = 0
+    ghost_src: np.ndarray | None = None   # original id of each ghost image
+    info: dict = field(default_factory=dict)
+
+    @property
+    def num_particles(self):
+        return self.particles.shape[0]
+
+    @property
+    def num_tets(self):
+        return self.tets.shape[0]
+
+
+def vert_to_tet(tets: np.ndarray, num_particles: int) -> np.ndarray:
+    """For each vertex the LAST tet (highest index) containing it, -1 if none
+    (src/tess.cpp:774-786: the last one wins)."""
+    v2t = np.full(num_particles, -1, np.int32)
+    T = tets.shape[0]
+    if T == 0:
+        return v2t
+    verts = tets[:, :4]
+    ids = np.broadcast_to(np.arange(T, dtype=np.int32)[:, None], verts.shape)
+    # the last tet containing a vertex is the one with the largest index
+    np.maximum.at(v2t, verts.ravel(), ids.ravel())
+    return v2t
+
+
+def qhull_tets(points: np.ndarray) -> np.ndarray:
+    """Delaunay tets of float32 points as an (T, 8) int32 tet_t array.
+    Points are promoted to double as the reference does (src/tess-qhull.c:39-42)."""
+    from scipy.spatial import Delaunay
+    p64 = np.ascontiguousarray(points, dtype=np.float32).astype(np.float64)
+    d = Delaunay(p64, qhull_options="Qt")
+    T = d.simplices.shape[0]
+    tets = np.empty((T, 8), np.int32)
+    tets[:, :4] = d.simplices
+    tets[:, 4:] = d.neighbors
+    return tets
+
+
+def periodic_ghosts(p: np.ndarray, width: float, lo=0.0, hi=1.0):
+    """Periodic images of p within `width` of the box [lo, hi)^3, shifted in
+    float32 as wrap_pt does (src/tess.cpp:703-710).  Returns (ghosts, src)."""
+    p = np.ascontiguousarray(p, dtype=np.float32)
+    ext = np.float32(hi - lo)
+    w_lo = np.float32(lo - width)
+    w_hi = np.float32(hi + width)
+    gs, src = [], []
+    ids = np.arange(p.shape[0], dtype=np.int64)
+    # candidates: only particles within width of some face can have images
+    near = np.any((p < np.float32(lo + width)) | (p > np.float32(hi - width)), axis=1)
+    pn, idn = p[near], ids[near]
+    for sx in (-1, 0, 1):
+        for sy in (-1, 0, 1):
+            for sz in (-1, 0, 1):
+                if sx == sy == sz == 0:
+                    continue
+                s = np.array([sx, sy, sz], np.float32) * ext
+                q = (pn + s).astype(np.float32)
+                m = np.all((q >= w_lo) & (q <= w_hi), axis=1)
+                gs.append(q[m])
+                src.append(idn[m])
+    if gs:
+        return np.concatenate(gs).astype(np.float32), np.concatenate(src)
+    return np.zeros((0, 3), np.float32), np.zeros(0, np.int64)
+
+
+def _circumspheres(pts64: np.ndarray, tv: np.ndarray):
+    """Circumcenters and squared radii in float64 (for the slab acceptance test)."""
+    a = pts64[tv[:, 0]]
+    b = pts64[tv[:, 1]] - a
+    c = pts64[tv[:, 2]] - a
+    d = pts64[tv[:, 3]] - a
+    bb = (b * b).sum(1)
+    cc = (c * c).sum(1)
+    dd = (d * d).sum(1)
+    cxd = np.cross(c, d)
+    dxb = np.cross(d, b)
+    bxc = np.cross(b, c)
+    den = 2.0 * (b * cxd).sum(1)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        off = (bb[:, None] * cxd + cc[:, None] * dxb + dd[:, None] * bxc) / den[:, None]
+    r2 = (off * off).sum(1)
+    return a + off, r2
+
+
+def _column_job(args):
+    """Delaunay of one x-y column plus its margin; keeps the tets the column
+    owns (lowest-id vertex inside it) whose circumsphere, clipped to the
+    point cloud's bounding box, lies inside the padded column: no point
+    outside the padded column can then lie in the sphere, so the tet is
+    Delaunay for the whole set."""
+    (pts, lo, hi, margin, cloud_lo, cloud_hi) = args
+    pad_lo = lo - margin
+    pad_hi = hi + margin
+    inside = np.ones(pts.shape[0], bool)
+    for ax in (0, 1):
+        inside &= (pts[:, ax] >= pad_lo[ax]) & (pts[:, ax] < pad_hi[ax])
+    sel = np.nonzero(inside)[0]
+    sub = pts[sel]
+    t0 = time.time()
+    tets = qhull_tets(sub)
+    tv = sel[tets[:, :4]]                        # global vertex ids
+    ov = pts[tv.min(1)]
+    own = np.ones(tv.shape[0], bool)
+    for ax in (0, 1):
+        own &= (ov[:, ax] >= lo[ax]) & (ov[:, ax] < hi[ax])
+    tv = tv[own]
+    cen, r2 = _circumspheres(pts.astype(np.float64), tv)
+    r = np.sqrt(r2)
+    # distance of the centre to the cloud box along each axis
+    gap = np.maximum(np.maximum(cloud_lo - cen, cen - cloud_hi), 0.0)
+    ok = np.isfinite(r)
+    for ax in (0, 1):
+        # half-width along ax of the sphere's intersection with the box
+        rest = r2 - sum(gap[:, o] ** 2 for o in range(3) if o != ax)
+        hw = np.sqrt(np.maximum(rest, 0.0))
+        smin = np.maximum(cen[:, ax] - hw, cloud_lo[ax])
+        smax = np.minimum(cen[:, ax] + hw, cloud_hi[ax])
+        ok &= (smin >= pad_lo[ax]) & (smax <= pad_hi[ax])
+    return tv, ok, time.time() - t0, sub.shape[0]
+
+
+def _rebuild_neighbors(tv: np.ndarray) -> np.ndarray:
+    """tets[t][i] = tet sharing the face opposite verts[i] (-1 if none)."""
+    T = tv.shape[0]
+    faces = np.empty((T, 4, 3), np.int64)
+    for i in range(4):
+        faces[:, i] = np.delete(tv, i, axis=1)
+    faces.sort(axis=2)
+    f = faces.reshape(-1, 3)
+    n = np.int64(int(tv.max()) + 1)
+    key = (f[:, 0] * n + f[:, 1]) * n + f[:, 2]
+    order = np.argsort(key, kind="stable")
+    ks = key[order]
+    nb = np.full(T * 4, -1, np.int64)
+    same = ks[1:] == ks[:-1]
+    i0 = order[:-1][same]
+    i1 = order[1:][same]
+    if np.any(ks[2:][same[1:] & same[:-1]] == ks[1:-1][same[1:] & same[:-1]]):
+        raise RuntimeError("face shared by more than two tets: inconsistent slabs")
+    nb[i0] = i1 // 4
+    nb[i1] = i0 // 4
+    return nb.reshape(T, 4).astype(np.int32)
+
+
+def open_star_vertices(tets: np.ndarray, num_particles: int) -> np.ndarray:
+    """Mask of vertices on a boundary face (a face with no neighbour): exactly
+    the vertices whose star is not closed, i.e. complete() would fail."""
+    m = np.zeros(num_particles, bool)
+    for i in range(4):
+        b = tets[:, 4 + i] < 0
+        for j in range(4):
+            if j != i:
+                m[tets[b, j]] = True
+    return m
+
+
+def tessellate(points: np.ndarray, workers: int = 1, columns: int | None = None,
+               log=None, min_points: int = 200_000, closed_first: int = 0) -> np.ndarray:
+    """Delaunay tets (T, 8) of all points.  With workers > 1 the points are
+    split into x-y columns tessellated in parallel processes and merged into
+    one tessellation (tet numbering then follows the columns).  Only tets
+    certified Delaunay for the whole set are kept; the first `closed_first`
+    points (the original particles of a ghosted block) must end up with
+    closed stars, otherwise the margin doubles and the columns are redone."""
+    points = np.ascontiguousarray(points, dtype=np.float32)
+    n = points.shape[0]
+    if workers <= 1 or n < min_points:
+        return qhull_tets(points)
+    from concurrent.futures import ProcessPoolExecutor
+    cols = columns or workers
+    nx = int(np.floor(np.sqrt(cols)))
+    while cols % nx:
+        nx -= 1
+    ny = cols // nx
+    qx = np.quantile(points[:, 0], np.linspace(0, 1, nx + 1))
+    qy = np.quantile(points[:, 1], np.linspace(0, 1, ny + 1))
+    qx[0], qx[-1], qy[0], qy[-1] = -np.inf, np.inf, -np.inf, np.inf
+    cloud_lo = points.min(0).astype(np.float64)
+    cloud_hi = points.max(0).astype(np.float64)
+    extent = float((cloud_hi - cloud_lo).max())
+    spacing = extent / max(n, 1) ** (1.0 / 3.0)
+    margin = 3.0 * spacing
+    for attempt in range(5):
+        jobs = [(points, np.array([qx[i], qy[j]]), np.array([qx[i + 1], qy[j + 1]]),
+                 margin, cloud_lo, cloud_hi) for i in range(nx) for j in range(ny)]
+        parts, dropped = [], 0
+        with ProcessPoolExecutor(max_workers=workers) as ex:
+            for k, (tv, ok, dt, m) in enumerate(ex.map(_column_job, jobs)):
+                if log:
+                    log(f"  column {k + 1}/{len(jobs)}: {m} points, {int(ok.sum())} tets, "
+                        f"{dt:.1f}s")
+                dropped += int((~ok).sum())
+                parts.append(tv[ok])
+        tv = np.concatenate(parts).astype(np.int32)
+        nb = _rebuild_neighbors(tv)
+        tets = np.empty((tv.shape[0], 8), np.int32)
+        tets[:, :4] = tv
+        tets[:, 4:] = nb
+        if closed_first <= 0 and dropped == 0:
+            return tets
+        if closed_first > 0:
+            bad = int(open_star_vertices(tets, n)[:closed_first].sum())
+            if bad == 0:
+                if log:
+                    log(f"  merged {tets.shape[0]} tets ({dropped} uncertified hull-side "
+                        f"tets left out); all {closed_first} original stars closed")
+                return tets
+            if log:
+                log(f"  {bad} original stars open; margin x2")
+        margin *= 2.0
+    raise RuntimeError("column tessellation did not converge")
+
+
+def make_block(particles: np.ndarray, num_orig: int, bounds_min, bounds_max,
+               workers: int = 1, gid: int = 0, ghost_src=None, log=None,
+               closed_first: int = 0) -> HostBlock:
+    particles = np.ascontiguousarray(particles, dtype=np.float32)
+    t0 = time.time()
+    tets = tessellate(particles, workers=workers, log=log, closed_first=closed_first)
+    t1 = time.time()
+    v2t = vert_to_tet(tets, particles.shape[0])
+    return HostBlock(particles=particles, tets=tets, vert_to_tet=v2t, num_orig=num_orig,
+                     bounds_min=np.asarray(bounds_min, np.float32),
+                     bounds_max=np.asarray(bounds_max, np.float32), gid=gid,
+                     ghost_src=ghost_src, info=dict(qhull_s=t1 - t0))
+
+
+def periodic_block(p: np.ndarray, ghost_width: float, workers: int = 1,
+                   log=None) -> HostBlock:
+    """One periodic unit-cube block: originals + periodic ghost images."""
+    p = np.ascontiguousarray(p, dtype=np.float32)
+    g, src = periodic_ghosts(p, ghost_width)
+    allp = np.concatenate([p, g]).astype(np.float32)
+    return make_block(allp, p.shape[0], (0, 0, 0), (1, 1, 1), workers=workers,
+                      ghost_src=src, log=log, closed_first=p.shape[0])
+
+
+def default_workers() -> int:
+    n = os.cpu_count() or 1
+    return max(1, min(16, n))

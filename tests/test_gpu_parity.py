@@ -1,0 +1,127 @@
+"""GPU parity: the HIP library against the oracle, through the C ABI.
+
+Bar: bit-exact (float32 bit patterns, per-cell status and point counts) for
+circumcenters, volumes and densities of the reference-scan estimator and of
+the CIC estimator.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from tess2_amd import host as D
+from tess2_amd import ingest, synthetic
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def bits_equal(a, b):
+    a = np.ascontiguousarray(a, np.float32)
+    b = np.ascontiguousarray(b, np.float32)
+    return a.shape == b.shape and np.array_equal(a.view(np.uint32), b.view(np.uint32))
+
+
+def first_diff(a, b):
+    a = np.ascontiguousarray(a, np.float32).ravel()
+    b = np.ascontiguousarray(b, np.float32).ravel()
+    d = np.nonzero(a.view(np.uint32) != b.view(np.uint32))[0]
+    return (len(d), d[:5].tolist(), a[d[:5]].tolist(), b[d[:5]].tolist())
+
+
+@pytest.fixture(scope="module")
+def blocks():
+    out = {}
+    out["periodic4k"] = ingest.periodic_block(synthetic.uniform(4096, seed=31), 4.0 / 16)
+    rng = np.random.default_rng(1234)
+    p = rng.random((4096, 3)).astype(np.float32)
+    out["p2"] = ingest.make_block(p, 4096, (0, 0, 0), (1, 1, 1))
+    out["grf"] = ingest.periodic_block(synthetic.grf_lattice(24, 0.3, seed=5), 4.0 / 24)
+    return out
+
+
+@pytest.mark.parametrize("name", ["periodic4k", "p2", "grf"])
+def test_circumcenters_bitexact(oracle, blocks, name):
+    hb = blocks[name]
+    say("circumcenters", name, hb.num_tets)
+    got = D.fill_circumcenters(hb.tets, hb.particles)
+    want = oracle.circumcenters(hb.tets, hb.particles)
+    assert bits_equal(got, want), first_diff(got, want)
+
+
+@pytest.mark.parametrize("name", ["periodic4k", "p2", "grf"])
+def test_volumes_bitexact(oracle, blocks, name):
+    hb = blocks[name]
+    say("volumes", name, hb.num_orig)
+    got = D.volumes(hb)
+    want = oracle.volumes(oracle.Block.from_host(hb))
+    assert bits_equal(got, want), first_diff(got, want)
+
+
+def _oracle_dense(oracle, hb, g, cic=False, mass=1.0):
+    ob = oracle.Block.from_host(hb)
+    return oracle.dense_single(ob, g, cic=cic, mass=mass)
+
+
+@pytest.mark.parametrize("name,g", [("periodic4k", 32), ("periodic4k", 64), ("p2", 64),
+                                    ("periodic4k", 8), ("grf", 40)])
+def test_dense_reference_scan_bitexact(oracle, blocks, name, g):
+    hb = blocks[name]
+    say("dense", name, g)
+    res = D.dense(D.DENSE_TESS, 0, None, None, False, None, 1.0, 1e-4, (g, g, g), [hb],
+                  want_cells=True)
+    want = _oracle_dense(oracle, hb, (g, g, g))
+    st, npts = res.cells[0]
+    assert np.array_equal(st, want["status"])
+    assert np.array_equal(npts, want["npts"])
+    assert bits_equal(res.density[0].ravel(), want["density"]), \
+        first_diff(res.density[0], want["density"])
+    assert res.stats["points_offblock"] == len(want["offblock"][1])
+
+
+def test_dense_cic_estimator_bitexact(oracle, blocks):
+    hb = blocks["periodic4k"]
+    say("dense cic")
+    res = D.dense(D.DENSE_CIC, 0, None, None, False, None, 2.0, 1e-4, (24, 24, 24), [hb])
+    want = _oracle_dense(oracle, hb, (24, 24, 24), cic=True, mass=2.0)
+    assert bits_equal(res.density[0].ravel(), want["density"]), \
+        first_diff(res.density[0], want["density"])
+
+
+def test_staged_runs_repeat_identically(oracle, blocks):
+    hb = blocks["grf"]
+    say("staged repeat")
+    run = D.StagedRun([hb])
+    g = (48, 48, 48)
+    outs = []
+    for _ in range(3):
+        run.run(g)
+        st = run.sync()
+        outs.append(run.fetch(0, ngrid=48 ** 3, volumes=True)["density"])
+    want = _oracle_dense(oracle, hb, g)
+    for o in outs:
+        assert bits_equal(o, want["density"])
+    assert st["cells_scan"] + st["cells_cic"] == int((want["status"] <= 1).sum())
+
+
+def test_tiny_and_empty_blocks(oracle):
+    say("tiny")
+    # 5 points: one interior cell (complete), four hull cells
+    p = np.float32([[.5, .5, .5], [.1, .1, .1], [.9, .1, .2], [.2, .9, .15], [.5, .5, .95],
+                    [.15, .2, .9], [.9, .9, .9], [.85, .15, .85], [.1, .85, .8]])
+    hb = ingest.make_block(p, p.shape[0], (0, 0, 0), (1, 1, 1))
+    res = D.dense(D.DENSE_TESS, 0, None, None, False, None, 1.0, 1e-4, (16, 16, 16), [hb],
+                  want_cells=True)
+    want = _oracle_dense(oracle, hb, (16, 16, 16))
+    assert np.array_equal(res.cells[0][0], want["status"])
+    assert bits_equal(res.density[0].ravel(), want["density"])
+    assert bits_equal(D.volumes(hb), oracle.volumes(oracle.Block.from_host(hb)))
+    # a block with particles but no original cells
+    hb0 = ingest.HostBlock(p, hb.tets, hb.vert_to_tet, 0, np.float32([0, 0, 0]),
+                           np.float32([1, 1, 1]))
+    res0 = D.dense(D.DENSE_TESS, 0, None, None, False, None, 1.0, 1e-4, (8, 8, 8), [hb0])
+    assert not res0.density[0].any()

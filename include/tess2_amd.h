@@ -176,6 +176,10 @@ int tess2_sync(tess2_ctx *ctx, tess2_stats_t *stats);
 int tess2_fetch(tess2_ctx *ctx, int b, float *density, float *volumes,
                 float *circumcenters);
 
+/* Nearest-site owner of every grid point of block b from the last
+ * EXACT_VORONOI run (-1 outside the block bounds); block grid order. */
+int tess2_fetch_owner(tess2_ctx *ctx, int b, int *owner);
+
 /* Average device time per launch (ms) of the dominant kernel over the
  * runs since the last call, measured with HIP events on the library
  * stream, and the number of launches averaged. */

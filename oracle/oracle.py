@@ -80,6 +80,9 @@ def lib():
                                           F32P, vp, vp, vp, C.c_long, vp]
         L.o_recvd_pts.argtypes = [I32P, F64P, C.c_long, I32P, I32P, C.POINTER(OArgs), F32P]
         L.o_exact_owner.argtypes = [F32P, C.c_int, I32P, F32P, F32P, I32P]
+        L.o_exact_owner_off.argtypes = [F32P, C.c_int, I32P, I32P, F32P, F32P, I32P]
+        L.o_exact_dense.argtypes = [C.POINTER(OBlock), C.POINTER(OArgs), I32P, I32P, I32P,
+                                    I32P, I32P, F64P]
         _lib = L
     return _lib
 
@@ -304,3 +307,22 @@ def dense_single(block, gnum, mass=1.0, eps=1e-4, cic=False, project=False,
     return dict(density=den, status=status, npts=npts, offblock=off, tot_mass=tm,
                 grid_phys_mins=gmin, grid_phys_maxs=gmax, grid_step_size=step,
                 block_min_idx=bmn, block_max_idx=bmx, block_num_idx=bnum, args=args)
+
+
+def exact_dense(block, gnum, mass=1.0, eps=1e-4):
+    """Exact-Voronoi density of one block whose bounds are the data bounds
+    (FP64 oracle of the exact assignment).  Returns a dict with owner (block
+    grid, z-y-x order), status, npts and density (float64)."""
+    dmin = np.array(block.cs.bounds_min[:], np.float32)
+    dmax = np.array(block.cs.bounds_max[:], np.float32)
+    gmin, gmax, step = grid_step_params(dmin, dmax, gnum)
+    bmn, bmx, bnum = block_grid_params(dmin, dmax, gmin, step, eps, dmin, dmax, gnum)
+    args = make_args(mass, dmin, dmax, gmin, step, eps, gnum)
+    ng = int(bnum[0]) * int(bnum[1]) * int(bnum[2])
+    owner = np.zeros(ng, np.int32)
+    status = np.zeros(block.num_orig, np.int32)
+    npts = np.zeros(block.num_orig, np.int32)
+    den = np.zeros(ng, np.float64)
+    lib().o_exact_dense(C.byref(block.cs), C.byref(args), bmn, bnum, owner, status, npts, den)
+    return dict(owner=owner, status=status, npts=npts, density=den, grid_step_size=step,
+                grid_phys_mins=gmin, block_num_idx=bnum, args=args)

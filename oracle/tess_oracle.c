@@ -905,6 +905,13 @@ static double d2_site(const float *g, const float *p)
 void o_exact_owner(const float *particles, int np, const int *gnum,
                    const float *step, const float *gmin, int *owner)
 {
+    const int off[3] = {0, 0, 0};
+    o_exact_owner_off(particles, np, off, gnum, step, gmin, owner);
+}
+
+void o_exact_owner_off(const float *particles, int np, const int *goff, const int *gnum,
+                       const float *step, const float *gmin, int *owner)
+{
     /* bucket hash over the particle bounding box, ~2 particles per bucket */
     double lo[3], hi[3];
     for (int i = 0; i < 3; ++i) {
@@ -953,7 +960,7 @@ void o_exact_owner(const float *particles, int np, const int *gnum,
     for (int z = 0; z < gnum[2]; ++z)
         for (int y = 0; y < gnum[1]; ++y)
             for (int x = 0; x < gnum[0]; ++x) {
-                int gi[3] = {x, y, z};
+                int gi[3] = {x + goff[0], y + goff[1], z + goff[2]};
                 float g[3];
                 o_idx2phys(gi, g, step, gmin);
                 int c[3];
@@ -1002,4 +1009,143 @@ void o_exact_owner(const float *particles, int np, const int *gnum,
             }
     free(start);
     free(items);
+}
+
+/* ------------------------------------------------------------------------ */
+/* exact-Voronoi density (FP64 oracle)                                       */
+/* ------------------------------------------------------------------------ */
+static void cc_f64(const o_tet_t *t, const float *P, double *o)
+{
+    const float *a = &P[3 * t->verts[0]], *b = &P[3 * t->verts[1]];
+    const float *c = &P[3 * t->verts[2]], *d = &P[3 * t->verts[3]];
+    double t0 = (double)a[0] - d[0], t1 = (double)a[1] - d[1], t2 = (double)a[2] - d[2];
+    double u0 = (double)b[0] - d[0], u1 = (double)b[1] - d[1], u2 = (double)b[2] - d[2];
+    double v0 = (double)c[0] - d[0], v1 = (double)c[1] - d[1], v2 = (double)c[2] - d[2];
+    double nt = t0 * t0 + t1 * t1 + t2 * t2;
+    double nu = u0 * u0 + u1 * u1 + u2 * u2;
+    double nv = v0 * v0 + v1 * v1 + v2 * v2;
+    double uv0 = u1 * v2 - u2 * v1, uv1 = u2 * v0 - u0 * v2, uv2 = u0 * v1 - u1 * v0;
+    double vt0 = v1 * t2 - v2 * t1, vt1 = v2 * t0 - v0 * t2, vt2 = v0 * t1 - v1 * t0;
+    double tu0 = t1 * u2 - t2 * u1, tu1 = t2 * u0 - t0 * u2, tu2 = t0 * u1 - t1 * u0;
+    double den = 2.0 * (t0 * uv0 + t1 * uv1 + t2 * uv2);
+    o[0] = (double)d[0] + (nt * uv0 + nu * vt0 + nv * tu0) / den;
+    o[1] = (double)d[1] + (nt * uv1 + nu * vt1 + nv * tu1) / den;
+    o[2] = (double)d[2] + (nt * uv2 + nu * vt2 + nv * tu2) / den;
+}
+
+void o_exact_dense(const o_block_t *b, const o_args_t *a, const int *bmin_idx,
+                   const int *bnum, int *owner, int *status, int *npts, double *density)
+{
+    const int nc = b->num_orig_particles;
+    float lo[3], hi[3];
+    for (int i = 0; i < 3; ++i) {
+        float e = (a->data_maxs[i] - a->data_mins[i]) * 2.0f * FLT_EPSILON;
+        lo[i] = a->data_mins[i] - e;
+        hi[i] = a->data_maxs[i] + e;
+    }
+    /* 1. which cells deposit */
+    for (int c = 0; c < nc; ++c) {
+        int vt = b->vert_to_tet[c];
+        if (vt < 0 || !star_bfs(c, b->tets, vt, &g_q, &g_vis, &g_vv, NULL, NULL)) {
+            status[c] = O_CELL_INCOMPLETE;
+            continue;
+        }
+        /* flat (coplanar, cocircular) tets have no circumcenter: skipped */
+        double mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
+        int defined = 0;
+        for (int k = 0; k < g_vis.n; ++k) {
+            double o[3];
+            cc_f64(&b->tets[g_vis.p[k]], b->particles, o);
+            if (isnan(o[0]) || isnan(o[1]) || isnan(o[2]))
+                continue;
+            defined++;
+            for (int i = 0; i < 3; ++i) {
+                if (o[i] < mn[i]) mn[i] = o[i];
+                if (o[i] > mx[i]) mx[i] = o[i];
+            }
+        }
+        if (!defined)
+            status[c] = O_CELL_DEGENERATE;
+        else if (mn[0] < lo[0] || mn[1] < lo[1] || mn[2] < lo[2] || mx[0] > hi[0] ||
+                 mx[1] > hi[1] || mx[2] > hi[2])
+            status[c] = O_CELL_OUTSIDE;
+        else
+            status[c] = O_CELL_SCAN;
+    }
+    /* 2. nearest site of every block grid point */
+    const long ng = (long)bnum[0] * bnum[1] * bnum[2];
+    o_exact_owner_off(b->particles, b->num_particles, bmin_idx, bnum, a->grid_step_size,
+                      a->grid_phys_mins, owner);
+    for (int c = 0; c < nc; ++c)
+        npts[c] = 0;
+    for (long g = 0; g < ng; ++g) {
+        int x = (int)(g % bnum[0]), y = (int)((g / bnum[0]) % bnum[1]),
+            z = (int)(g / ((long)bnum[0] * bnum[1]));
+        int gi[3] = {bmin_idx[0] + x, bmin_idx[1] + y, bmin_idx[2] + z};
+        float pos[3];
+        o_idx2phys(gi, pos, a->grid_step_size, a->grid_phys_mins);
+        if (!(pos[0] >= b->bounds_min[0] && pos[0] <= b->bounds_max[0] &&
+              pos[1] >= b->bounds_min[1] && pos[1] <= b->bounds_max[1] &&
+              pos[2] >= b->bounds_min[2] && pos[2] <= b->bounds_max[2])) {
+            owner[g] = -1;
+            continue;
+        }
+        int o = owner[g];
+        if (o >= 0 && o < nc && status[o] == O_CELL_SCAN)
+            npts[o]++;
+    }
+    /* 3. density: contributions in ascending cell order */
+    const double div = (double)(a->grid_step_size[0] * a->grid_step_size[1] *
+                                a->grid_step_size[2]);
+    for (long g = 0; g < ng; ++g)
+        density[g] = 0.0;
+    /* owner contributions: each point has at most one */
+    double *own_part = (double *)calloc((size_t)ng, sizeof(double));
+    int *own_cell = (int *)malloc(sizeof(int) * (size_t)ng);
+    for (long g = 0; g < ng; ++g) {
+        int o = owner[g];
+        own_cell[g] = -1;
+        if (o >= 0 && o < nc && status[o] == O_CELL_SCAN && npts[o] > 0) {
+            own_part[g] = (double)a->mass / (double)npts[o];
+            own_cell[g] = o;
+        }
+    }
+    /* CIC contributions of deposited cells without points, in cell order,
+     * merged with the owner contribution at its cell position */
+    for (int c = 0; c < nc; ++c) {
+        if (status[c] != O_CELL_SCAN || npts[c] > 0)
+            continue;
+        status[c] = O_CELL_CIC;
+        npts[c] = 8;
+        int i24[24];
+        float m8[8];
+        o_distribute_cic(&b->particles[3 * c], a->mass, i24, m8, a->grid_step_size,
+                         a->grid_phys_mins, a->eps);
+        for (int k = 0; k < 8; ++k) {
+            float pos[3];
+            o_idx2phys(&i24[3 * k], pos, a->grid_step_size, a->grid_phys_mins);
+            if (!(pos[0] >= b->bounds_min[0] && pos[0] <= b->bounds_max[0] &&
+                  pos[1] >= b->bounds_min[1] && pos[1] <= b->bounds_max[1] &&
+                  pos[2] >= b->bounds_min[2] && pos[2] <= b->bounds_max[2]))
+                continue;
+            int l[3] = {i24[3 * k] - bmin_idx[0], i24[3 * k + 1] - bmin_idx[1],
+                        i24[3 * k + 2] - bmin_idx[2]};
+            if (l[0] < 0 || l[1] < 0 || l[2] < 0 || l[0] >= bnum[0] || l[1] >= bnum[1] ||
+                l[2] >= bnum[2])
+                continue;
+            long g = ((long)l[2] * bnum[1] + l[1]) * bnum[0] + l[0];
+            if (own_cell[g] >= 0 && own_cell[g] < c) {
+                density[g] += own_part[g];
+                own_cell[g] = -2;     /* folded */
+            }
+            density[g] += (double)m8[k];
+        }
+    }
+    for (long g = 0; g < ng; ++g) {
+        if (own_cell[g] >= 0)
+            density[g] += own_part[g];
+        density[g] /= div;
+    }
+    free(own_part);
+    free(own_cell);
 }

@@ -144,6 +144,25 @@ void o_recvd_pts(const int *idx3, const double *mass, long n,
 void o_exact_owner(const float *particles, int num_particles, const int *gnum,
                    const float *step, const float *gmin, int *owner);
 
+/* Same, for the sub-grid of global indices [off, off + gnum) (a block grid). */
+void o_exact_owner_off(const float *particles, int num_particles, const int *off,
+                       const int *gnum, const float *step, const float *gmin, int *owner);
+
+/* Exact-Voronoi density of one block (FP64 oracle of the exact mode).
+ * A cell deposits when it is complete and the FP64 bounding box of its
+ * Voronoi vertices (circumcenters of its star, solved in double; flat tets,
+ * whose circumcenter is undefined (NaN), are skipped, and a cell with none
+ * defined is degenerate) stays inside the data bounds with the reference's
+ * 2*FLT_EPSILON*extent slack.  Every
+ * in-block grid point goes to its nearest site (o_exact_owner); a deposited
+ * cell with N points gives each mass/N, one with none spreads its mass by
+ * the reference's CIC.  Contributions are summed in double in ascending cell
+ * order; density = sum / div.  owner (block grid) gets the nearest site of
+ * each in-block point (-1 outside the block), status/npts as O_CELL_*. */
+void o_exact_dense(const o_block_t *b, const o_args_t *a, const int *block_min_idx,
+                   const int *block_num_idx, int *owner, int *status, int *npts,
+                   double *density);
+
 #ifdef __cplusplus
 }
 #endif

@@ -77,6 +77,7 @@ SIGNATURES = {
     "tess2_sync": (C.c_int, [C.c_void_p, C.POINTER(Stats)]),
     "tess2_fetch": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]),
     "tess2_kernel_time": (C.c_int, [C.c_void_p, F32P, I32P]),
+    "tess2_fetch_owner": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p]),
 }
 
 

@@ -39,6 +39,14 @@ struct DevBlock {
     unsigned ovf_cap = 0;
     int *slow = nullptr;
     unsigned slow_cap = 0;
+    // exact-Voronoi assignment
+    int *owner = nullptr;                    // [grid] nearest site
+    int *status_ex = nullptr;                // [ncells] deposit decision
+    HashP hash{};
+    unsigned *hstart = nullptr, *hfill = nullptr, *hsums = nullptr;
+    int *hbk = nullptr;
+    float4 *hsorted = nullptr;
+    size_t owner_cap = 0;
     unsigned *ctrs = nullptr;                // C_NCTR
     unsigned long long *cstat = nullptr;     // 6 cell stats
     double *gsum = nullptr;
@@ -112,8 +120,10 @@ static int dalloc(tess2_ctx *c, T **p, size_t n)
 
 static void free_block(DevBlock &b)
 {
-    void *ps[] = {b.pts, b.tets, b.v2t, b.cc, b.vol, b.status, b.npts, b.density, b.count,
-                  b.first, b.head, b.ovf, b.slow, b.ctrs, b.cstat, b.gsum, b.gmax};
+    void *ps[] = {b.pts,    b.tets,   b.v2t,   b.cc,   b.vol,  b.status,    b.npts,
+                  b.density, b.count, b.first, b.head, b.ovf,  b.slow,      b.ctrs,
+                  b.cstat,  b.gsum,   b.gmax,  b.owner, b.status_ex, b.hstart, b.hfill,
+                  b.hsums,  b.hbk,    b.hsorted};
     for (void *p : ps)
         if (p) (void)hipFree(p);
     b = DevBlock();
@@ -281,6 +291,38 @@ extern "C" int tess2_stage(tess2_ctx *c, const tess2_block_t *blocks, int nblock
                             hipMemcpyHostToDevice));
         d.slow_cap = (unsigned)std::max(1024, d.norig / 32);
         if ((r = dalloc(c, &d.slow, d.slow_cap))) return r;
+        // bucket hash over the particle bounding box, about two particles
+        // per bucket; at most 2^24 - 1024 buckets, so that the bucket scan
+        // needs at most 16384 blocks of 1024 (k_scan_sums' limit)
+        float lo[3] = {0, 0, 0}, hi[3] = {1, 1, 1};
+        for (int p = 0; p < d.np; ++p)
+            for (int i = 0; i < 3; ++i) {
+                const float x = hb.particles[3 * p + i];
+                if (p == 0 || x < lo[i]) lo[i] = x;
+                if (p == 0 || x > hi[i]) hi[i] = x;
+            }
+        double vol = 1.0;
+        for (int i = 0; i < 3; ++i) vol *= std::max((double)hi[i] - lo[i], 1e-12);
+        double cell = std::cbrt(vol * 2.0 / std::max(d.np, 1));
+        for (;;) {
+            long long nb = 1;
+            for (int i = 0; i < 3; ++i)
+                nb *= (long long)((hi[i] - lo[i]) / cell) + 1;
+            if (nb <= (1ll << 24) - 1024) break;
+            cell *= 1.25;
+        }
+        d.hash.cell = (float)cell;
+        d.hash.inv = (float)(1.0 / cell);
+        for (int i = 0; i < 3; ++i) {
+            d.hash.lo[i] = lo[i];
+            d.hash.dims[i] = (int)((hi[i] - lo[i]) / cell) + 1;
+        }
+        const size_t nbk = (size_t)d.hash.dims[0] * d.hash.dims[1] * d.hash.dims[2];
+        if ((r = dalloc(c, &d.hstart, nbk + 1)) || (r = dalloc(c, &d.hfill, nbk + 1)) ||
+            (r = dalloc(c, &d.hsums, (nbk + 1 + 1023) / 1024)) ||
+            (r = dalloc(c, &d.hbk, (size_t)d.np)) || (r = dalloc(c, &d.hsorted, (size_t)d.np)) ||
+            (r = dalloc(c, &d.status_ex, (size_t)d.norig)))
+            return r;
     }
     return 0;
 }
@@ -386,7 +428,26 @@ static int enqueue_run(tess2_ctx *c, const RunArgs &a)
         launch_fill_u32(d.gmax, 1, 0u, s);
         BlockIn in = make_in(d);
         CellOut o = make_out(d, a.want_vol != 0);
-        if (a.alg == TESS2_DENSE_CIC) {
+        if (a.alg == TESS2_DENSE_TESS && a.assignment == TESS2_ASSIGN_EXACT_VORONOI) {
+            if (d.owner_cap < n) {
+                r = dalloc(c, &d.owner, n);
+                if (r) return r;
+                d.owner_cap = n;
+            }
+            launch_circumcenters(d.pts, d.tets, d.nt, d.cc, s);
+            if (a.want_vol) {
+                // volume() as the reference computes it
+                CellOut ov = make_out(d, true);
+                launch_cells_ref(in, gp, ov, 0, s);
+                launch_cells_ref_slow(in, gp, ov, 0, s);
+            }
+            launch_hash_build(d.pts, d.np, d.hash, d.hstart, d.hfill, d.hsums, d.hbk, d.hsorted, s);
+            launch_exact_status(in, gp, d.status_ex, a.precision, d.ctrs, s);
+            if (first_block) HIPCHK(c, hipEventRecord(c->ev_k0, s));
+            launch_owner(gp, d.hash, d.hstart, d.hsorted, d.owner, a.precision, d.ctrs, s);
+            if (first_block) HIPCHK(c, hipEventRecord(c->ev_k1, s));
+            launch_exact_tail(in, gp, d.owner, d.status_ex, o, a.precision, s);
+        } else if (a.alg == TESS2_DENSE_CIC) {
             if (first_block) HIPCHK(c, hipEventRecord(c->ev_k0, s));
             launch_cic(in, gp, o, s);
             if (first_block) HIPCHK(c, hipEventRecord(c->ev_k1, s));
@@ -422,8 +483,12 @@ extern "C" int tess2_run(tess2_ctx *c, int alg, int num_given_bounds, const floa
     if (!c || !glo_num_idx) return fail(c, TESS2_E_ARG, "null argument");
     if (alg != TESS2_DENSE_TESS && alg != TESS2_DENSE_CIC)
         return fail(c, TESS2_E_ARG, "unknown estimator %d", alg);
-    if (assignment != TESS2_ASSIGN_REFERENCE_SCAN)
-        return fail(c, TESS2_E_ARG, "assignment mode %d not available", assignment);
+    if (assignment != TESS2_ASSIGN_REFERENCE_SCAN && assignment != TESS2_ASSIGN_EXACT_VORONOI)
+        return fail(c, TESS2_E_ARG, "unknown assignment mode %d", assignment);
+    if (precision != TESS2_PREC_MIXED && precision != TESS2_PREC_FP64)
+        return fail(c, TESS2_E_ARG, "unknown precision %d", precision);
+    if (assignment == TESS2_ASSIGN_EXACT_VORONOI && project)
+        return fail(c, TESS2_E_ARG, "the exact assignment has no projection mode");
     if (project) {
         // the reference's index() handles the x-y plane only (src/dense.cpp:1076-1077)
         if (!proj_plane || proj_plane[0] != 0.0f || proj_plane[1] != 0.0f ||
@@ -540,6 +605,19 @@ extern "C" int tess2_fetch(tess2_ctx *c, int b, float *density, float *volumes,
             circumcenters[3 * t + 2] = tmp[t].z;
         }
     }
+    return 0;
+}
+
+extern "C" int tess2_fetch_owner(tess2_ctx *c, int b, int *owner)
+{
+    if (!c || !owner || b < 0 || b >= (int)c->blocks.size())
+        return fail(c, TESS2_E_ARG, "bad arguments");
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    DevBlock &d = c->blocks[b];
+    if (!d.owner || d.owner_cap < d.ngrid)
+        return fail(c, TESS2_E_ARG, "no exact-assignment run on block %d", b);
+    HIPCHK(c, hipMemcpy(owner, d.owner, sizeof(int) * d.ngrid, hipMemcpyDeviceToHost));
     return 0;
 }
 

@@ -15,7 +15,9 @@ __global__ void k_circumcenters(const float4 *__restrict__ pts, const int4 *__re
     const float4 a = pts[vv.x], b = pts[vv.y], c = pts[vv.z], d = pts[vv.w];
     float x, y, z;
     circumcenter(a.x, a.y, a.z, b.x, b.y, b.z, c.x, c.y, c.z, d.x, d.y, d.z, x, y, z);
-    cc[t] = make_float4(x, y, z, 0.0f);
+    const float e = circumcenter_err(a.x, a.y, a.z, b.x, b.y, b.z, c.x, c.y, c.z, d.x, d.y, d.z,
+                                     x, y, z);
+    cc[t] = make_float4(x, y, z, e);
 }
 
 void launch_circumcenters(const float4 *pts, const int4 *tets, int ntets, float4 *cc,

@@ -1,0 +1,79 @@
+// deposit.hpp -- the grid deposit shared by the estimators: in-block test,
+// local index, and the ordered-accumulation protocol (first contribution
+// written directly, later ones listed for the ordered fold).
+#pragma once
+
+#include "geom.hpp"
+#include "kernels.hpp"
+
+namespace t2 {
+
+// ---------------------------------------------------------------------------
+// deposit one grid point (src/dense.cpp:276-305)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void deposit_point(int gx, int gy, int gz, float m, int cell,
+                                              const GridP &gp, const CellOut &o)
+{
+    const float px = idx2phys1(gx, gp.step[0], gp.gmin[0]);
+    const float py = idx2phys1(gy, gp.step[1], gp.gmin[1]);
+    const float pz = idx2phys1(gz, gp.step[2], gp.gmin[2]);
+    if (!(px >= gp.bmin[0] && px <= gp.bmax[0] && py >= gp.bmin[1] && py <= gp.bmax[1] &&
+          pz >= gp.bmin[2] && pz <= gp.bmax[2])) {
+        atomicAdd(&o.ctrs[C_OFFBLOCK], 1u);
+        return;
+    }
+    const int lx = gx - gp.bmin_idx[0], ly = gy - gp.bmin_idx[1], lz = gz - gp.bmin_idx[2];
+    if (lx < 0 || ly < 0 || lx >= gp.bnum_idx[0] || ly >= gp.bnum_idx[1] ||
+        (!gp.project && (lz < 0 || lz >= gp.bnum_idx[2]))) {
+        atomicAdd(&o.ctrs[C_OFFGRID], 1u);
+        return;
+    }
+    const long long lidx =
+        gp.project ? (long long)ly * gp.bnum_idx[0] + lx
+                   : ((long long)lz * gp.bnum_idx[1] + ly) * gp.bnum_idx[0] + lx;
+    const unsigned c = atomicAdd(&o.count[lidx], 1u);
+    if (c == 0u) {
+        o.first[lidx] = make_uint2((unsigned)cell, __float_as_uint(m));
+        o.density[lidx] = (float)((double)m / (double)gp.div);
+    } else {
+        const unsigned e = atomicAdd(&o.ctrs[C_OVF], 1u);
+        if (e < o.ovf_cap) {
+            const int prev = atomicExch(&o.head[lidx], (int)e);
+            o.ovf[e] = make_int4((int)lidx, cell, __float_as_int(m), prev);
+        }
+    }
+}
+
+// DistributeScalarCIC(): src/dense.cpp:1787-1872, corner k in z,y,x order.
+__device__ void cic_weights(float sx, float sy, float sz, float scalar, const GridP &gp,
+                            int mn[3], float m8[8])
+{
+    mn[0] = phys2idx1(sx, gp.step[0], gp.gmin[0]);
+    mn[1] = phys2idx1(sy, gp.step[1], gp.gmin[1]);
+    mn[2] = phys2idx1(sz, gp.step[2], gp.gmin[2]);
+    float tot = 0.0f, v0 = 0.0f;
+    const float e2 = 2.0f * gp.eps;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const int ix = mn[0] + (k & 1), iy = mn[1] + ((k >> 1) & 1), iz = mn[2] + (k >> 2);
+        const float gx = idx2phys1(ix, gp.step[0], gp.gmin[0]);
+        const float gy = idx2phys1(iy, gp.step[1], gp.gmin[1]);
+        const float gz = idx2phys1(iz, gp.step[2], gp.gmin[2]);
+        float px = sx, py = sy, pz = sz;
+        if (fabsf(px - gx) < gp.eps) px += e2;
+        if (fabsf(py - gy) < gp.eps) py += e2;
+        if (fabsf(pz - gz) < gp.eps) pz += e2;
+        const float vol = fabsf((gx - px) * (gy - py) * (gz - pz));
+        if (v0 == 0.0f) v0 = vol;
+        const float w = v0 / vol;
+        m8[k] = w;
+        tot += w;
+    }
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const float w = m8[k] / tot;
+        m8[k] = w * scalar;
+    }
+}
+
+}  // namespace t2

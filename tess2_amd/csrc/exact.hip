@@ -1,0 +1,576 @@
+// exact.hip -- the exact-Voronoi assignment: every grid point goes to its
+// nearest site (ties to the lowest particle id), located through a uniform
+// bucket hash of the particles built on the device for each run.
+//
+// Precision modes:
+//   MIXED: squared distances in float32; the FP32 decision stands only when
+//          the runner-up is outside a relative window of 32 ulp around the
+//          winner (each FP32 squared distance is within ~5 ulp of exact), so
+//          the FP32 choice equals the exact and the FP64 choice; otherwise
+//          the point is searched again in double (counted as a fallback).
+//          Deposit decisions of cells near the data-bounds threshold, or with
+//          ill-conditioned circumcenters, are re-made in double the same way.
+//   FP64:  every distance, circumcenter and bound test in double.
+#include "kernels.hpp"
+#include "geom.hpp"
+#include "deposit.hpp"
+
+namespace t2 {
+
+enum { EX_DEPOSIT = 0, EX_INCOMPLETE = 2, EX_OUTSIDE = 3, EX_DEGENERATE = 4 };
+
+// ---------------------------------------------------------------------------
+// bucket hash: count, exclusive scan, scatter (particles sorted by bucket,
+// stored as float4 {x, y, z, id bits})
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int bucket_of(float4 p, const HashP &h)
+{
+    int c[3];
+    const float q[3] = {p.x, p.y, p.z};
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        int ci = (int)floorf((q[i] - h.lo[i]) * h.inv);
+        ci = ci < 0 ? 0 : (ci >= h.dims[i] ? h.dims[i] - 1 : ci);
+        c[i] = ci;
+    }
+    return (c[2] * h.dims[1] + c[1]) * h.dims[0] + c[0];
+}
+
+__global__ void k_hash_count(const float4 *__restrict__ pts, int np, HashP h,
+                             unsigned *__restrict__ cnt, int *__restrict__ bk)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= np) return;
+    const int b = bucket_of(pts[p], h);
+    bk[p] = b;
+    atomicAdd(&cnt[b], 1u);
+}
+
+// exclusive scan in three passes (1024 elements per block)
+__global__ void k_scan_local(const unsigned *__restrict__ in, unsigned *__restrict__ out,
+                             unsigned *__restrict__ sums, int n)
+{
+    __shared__ unsigned s[1024];
+    const int i = blockIdx.x * 1024 + threadIdx.x;
+    const unsigned x = i < n ? in[i] : 0u;
+    s[threadIdx.x] = x;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        const unsigned y = threadIdx.x >= (unsigned)off ? s[threadIdx.x - off] : 0u;
+        __syncthreads();
+        s[threadIdx.x] += y;
+        __syncthreads();
+    }
+    if (i < n) out[i] = s[threadIdx.x] - x;
+    if (threadIdx.x == 1023) sums[blockIdx.x] = s[1023];
+}
+
+__global__ void k_scan_sums(unsigned *__restrict__ sums, int nb)
+{
+    // one block of 1024 threads; nb <= 1024 * 16
+    __shared__ unsigned s[1024];
+    unsigned loc[16];
+    unsigned run = 0;
+    const int per = (nb + 1023) / 1024;
+    for (int k = 0; k < per; ++k) {
+        const int i = threadIdx.x * per + k;
+        const unsigned x = i < nb ? sums[i] : 0u;
+        loc[k] = run;
+        run += x;
+    }
+    s[threadIdx.x] = run;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        const unsigned y = threadIdx.x >= (unsigned)off ? s[threadIdx.x - off] : 0u;
+        __syncthreads();
+        s[threadIdx.x] += y;
+        __syncthreads();
+    }
+    const unsigned base = s[threadIdx.x] - run;
+    for (int k = 0; k < per; ++k) {
+        const int i = threadIdx.x * per + k;
+        if (i < nb) sums[i] = base + loc[k];
+    }
+}
+
+__global__ void k_scan_add(unsigned *__restrict__ out, const unsigned *__restrict__ sums, int n,
+                           unsigned *__restrict__ fill)
+{
+    const int i = blockIdx.x * 1024 + threadIdx.x;
+    if (i < n) {
+        const unsigned v = out[i] + sums[blockIdx.x];
+        out[i] = v;
+        fill[i] = v;
+    }
+}
+
+__global__ void k_hash_scatter(const float4 *__restrict__ pts, int np, const int *__restrict__ bk,
+                               unsigned *__restrict__ fill, float4 *__restrict__ sorted)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= np) return;
+    const unsigned slot = atomicAdd(&fill[bk[p]], 1u);
+    const float4 q = pts[p];
+    sorted[slot] = make_float4(q.x, q.y, q.z, __int_as_float(p));
+}
+
+void launch_hash_build(const float4 *pts, int np, const HashP &h, unsigned *start,
+                       unsigned *fill, unsigned *sums, int *bk, float4 *sorted, hipStream_t s)
+{
+    const int nbk = h.dims[0] * h.dims[1] * h.dims[2];
+    launch_fill_u32(start, (size_t)nbk + 1, 0u, s);
+    if (np > 0)
+        hipLaunchKernelGGL(k_hash_count, dim3((np + 255) / 256), dim3(256), 0, s, pts, np, h,
+                           start, bk);
+    const int n = nbk + 1;   // the extra slot turns into the total
+    const int nblk = (n + 1023) / 1024;
+    hipLaunchKernelGGL(k_scan_local, dim3(nblk), dim3(1024), 0, s, start, start, sums, n);
+    hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(1024), 0, s, sums, nblk);
+    hipLaunchKernelGGL(k_scan_add, dim3(nblk), dim3(1024), 0, s, start, sums, n, fill);
+    if (np > 0)
+        hipLaunchKernelGGL(k_hash_scatter, dim3((np + 255) / 256), dim3(256), 0, s, pts, np, bk,
+                           fill, sorted);
+}
+
+// ---------------------------------------------------------------------------
+// nearest site of a grid point
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ bool better64(double d, int id, double bd, int bid)
+{
+    return d < bd || (d == bd && id < bid);
+}
+
+// search in double; returns the owner id
+__device__ int nearest_f64(float gx, float gy, float gz, const HashP &h,
+                           const unsigned *__restrict__ start, const float4 *__restrict__ hp)
+{
+    int c[3];
+    const float q[3] = {gx, gy, gz};
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        int ci = (int)floorf((q[i] - h.lo[i]) * h.inv);
+        c[i] = ci < 0 ? 0 : (ci >= h.dims[i] ? h.dims[i] - 1 : ci);
+    }
+    double best = 1e300;
+    int bid = 0x7fffffff;
+    const int rmax = h.dims[0] + h.dims[1] + h.dims[2];
+    for (int r = 0; r <= rmax; ++r) {
+        for (int dz = -r; dz <= r; ++dz) {
+            const int z = c[2] + dz;
+            if (z < 0 || z >= h.dims[2]) continue;
+            for (int dy = -r; dy <= r; ++dy) {
+                const int y = c[1] + dy;
+                if (y < 0 || y >= h.dims[1]) continue;
+                const bool face = (dz == -r || dz == r || dy == -r || dy == r);
+                const int step = face ? 1 : 2 * r;
+                for (int dx = -r; dx <= r; dx += (step > 0 ? step : 1)) {
+                    const int x = c[0] + dx;
+                    if (x < 0 || x >= h.dims[0]) continue;
+                    const int b = (z * h.dims[1] + y) * h.dims[0] + x;
+                    for (unsigned s = start[b]; s < start[b + 1]; ++s) {
+                        const float4 p = hp[s];
+                        const double ex = (double)gx - (double)p.x;
+                        const double ey = (double)gy - (double)p.y;
+                        const double ez = (double)gz - (double)p.z;
+                        const double d = ex * ex + ey * ey + ez * ez;
+                        const int id = __float_as_int(p.w);
+                        if (better64(d, id, best, bid)) {
+                            best = d;
+                            bid = id;
+                        }
+                    }
+                }
+            }
+        }
+        // every bucket beyond shell r is at least r*h away
+        // (r - 0.001) cells: the bucket of a point near a bucket face can be
+        // off by ~1e-4 of a cell through the float bucket arithmetic
+        const double lb = ((double)r - 0.001) * (double)h.cell;
+        if (bid != 0x7fffffff && r > 0 && lb * lb > best * (1.0 + 1e-9) + 1e-300) break;
+    }
+    return bid;
+}
+
+// search in float with the tie window; returns owner or -1 when undecided
+__device__ int nearest_f32(float gx, float gy, float gz, const HashP &h,
+                           const unsigned *__restrict__ start, const float4 *__restrict__ hp)
+{
+    int c[3];
+    const float q[3] = {gx, gy, gz};
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        int ci = (int)floorf((q[i] - h.lo[i]) * h.inv);
+        c[i] = ci < 0 ? 0 : (ci >= h.dims[i] ? h.dims[i] - 1 : ci);
+    }
+    float best = INFINITY, second = INFINITY;
+    int bid = -1;
+    const int rmax = h.dims[0] + h.dims[1] + h.dims[2];
+    for (int r = 0; r <= rmax; ++r) {
+        for (int dz = -r; dz <= r; ++dz) {
+            const int z = c[2] + dz;
+            if (z < 0 || z >= h.dims[2]) continue;
+            for (int dy = -r; dy <= r; ++dy) {
+                const int y = c[1] + dy;
+                if (y < 0 || y >= h.dims[1]) continue;
+                const bool face = (dz == -r || dz == r || dy == -r || dy == r);
+                const int step = face ? 1 : 2 * r;
+                for (int dx = -r; dx <= r; dx += (step > 0 ? step : 1)) {
+                    const int x = c[0] + dx;
+                    if (x < 0 || x >= h.dims[0]) continue;
+                    const int b = (z * h.dims[1] + y) * h.dims[0] + x;
+                    for (unsigned s = start[b]; s < start[b + 1]; ++s) {
+                        const float4 p = hp[s];
+                        const float ex = gx - p.x, ey = gy - p.y, ez = gz - p.z;
+                        const float d = ex * ex + ey * ey + ez * ez;
+                        if (d < best) {
+                            second = best;
+                            best = d;
+                            bid = __float_as_int(p.w);
+                        } else if (d < second) {
+                            second = d;
+                        }
+                    }
+                }
+            }
+        }
+        const float lb = ((float)r - 0.001f) * h.cell;
+        if (bid >= 0 && r > 0 && lb * lb > best * 1.0001f + 1e-30f) break;
+    }
+    // decided only if the runner-up is clearly farther (32 ulp window) and
+    // nothing underflowed
+    const float tau = 32.0f * 5.9604645e-8f;
+    if (bid >= 0 && best > 1e-30f && second > best * (1.0f + tau)) return bid;
+    if (bid >= 0 && best == 0.0f && second > 1e-30f) {
+        // the grid point coincides with a site: exact in float
+        return bid;
+    }
+    return -1;
+}
+
+// one thread per block grid point; a 64-lane wave covers a 4x4x4 tile
+__global__ __launch_bounds__(256) void k_owner(GridP gp, HashP h,
+                                               const unsigned *__restrict__ start,
+                                               const float4 *__restrict__ hp, int *__restrict__ owner,
+                                               int prec, unsigned *ctrs, int tiles_x, int tiles_y,
+                                               int ntiles)
+{
+    const int wave = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (wave >= ntiles) return;
+    const int lane = threadIdx.x & 63;
+    const int tx = wave % tiles_x, ty = (wave / tiles_x) % tiles_y, tz = wave / (tiles_x * tiles_y);
+    const int lx = tx * 4 + (lane & 3), ly = ty * 4 + ((lane >> 2) & 3), lz = tz * 4 + (lane >> 4);
+    if (lx >= gp.bnum_idx[0] || ly >= gp.bnum_idx[1] || lz >= gp.bnum_idx[2]) return;
+    const long long lidx = ((long long)lz * gp.bnum_idx[1] + ly) * gp.bnum_idx[0] + lx;
+    const int gx = gp.bmin_idx[0] + lx, gy = gp.bmin_idx[1] + ly, gz = gp.bmin_idx[2] + lz;
+    const float px = idx2phys1(gx, gp.step[0], gp.gmin[0]);
+    const float py = idx2phys1(gy, gp.step[1], gp.gmin[1]);
+    const float pz = idx2phys1(gz, gp.step[2], gp.gmin[2]);
+    if (!(px >= gp.bmin[0] && px <= gp.bmax[0] && py >= gp.bmin[1] && py <= gp.bmax[1] &&
+          pz >= gp.bmin[2] && pz <= gp.bmax[2])) {
+        owner[lidx] = -1;
+        return;
+    }
+    int o = -1;
+    if (prec == 0) o = nearest_f32(px, py, pz, h, start, hp);
+    if (o < 0) {
+        if (prec == 0) atomicAdd(&ctrs[C_FALLBACK], 1u);
+        o = nearest_f64(px, py, pz, h, start, hp);
+    }
+    owner[lidx] = o;
+}
+
+void launch_owner(const GridP &gp, const HashP &h, const unsigned *start, const float4 *hp,
+                  int *owner, int prec, unsigned *ctrs, hipStream_t s)
+{
+    const int tx = (gp.bnum_idx[0] + 3) / 4, ty = (gp.bnum_idx[1] + 3) / 4,
+              tz = (gp.bnum_idx[2] + 3) / 4;
+    const int ntiles = tx * ty * tz;
+    if (ntiles <= 0) return;
+    hipLaunchKernelGGL(k_owner, dim3((ntiles + 3) / 4), dim3(256), 0, s, gp, h, start, hp, owner,
+                       prec, ctrs, tx, ty, ntiles);
+}
+
+// ---------------------------------------------------------------------------
+// deposit status of each cell: complete + Voronoi-vertex bbox inside the
+// data bounds (one wave per cell; star BFS as in the reference-scan kernel)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int comp4(const int4 &q, int i)
+{
+    return i == 0 ? q.x : (i == 1 ? q.y : (i == 2 ? q.z : q.w));
+}
+
+template <int NS>
+__global__ __launch_bounds__(64) void k_exact_status(BlockIn in, GridP gp, int *status, int prec,
+                                                     unsigned *ctrs)
+{
+    constexpr int SMAX = 64 * NS;
+    __shared__ int4 s_v[SMAX];
+    __shared__ int4 s_n[SMAX];
+    const int v = blockIdx.x;
+    if (v >= in.ncells) return;
+    const int lane = threadIdx.x & 63;
+    const int vt = in.v2t[v];
+    if (vt < 0) {
+        if (lane == 0) status[v] = EX_INCOMPLETE;
+        return;
+    }
+    int sid[NS];
+#pragma unroll
+    for (int k = 0; k < NS; ++k) sid[k] = -1;
+    if (lane == 0) {
+        s_v[0] = in.tets[2 * vt];
+        s_n[0] = in.tets[2 * vt + 1];
+        sid[0] = vt;
+    }
+    int S = 1, f0 = 0, f1 = 1;
+    bool hull = false, over = false;
+    __syncthreads();
+    while (f0 < f1) {
+        const int ncand = (f1 - f0) * 4;
+        for (int base = 0; base < ncand; base += 64) {
+            const int key = base + lane;
+            int cand = -1;
+            bool valid = false, hh = false;
+            if (key < ncand) {
+                const int s = f0 + (key >> 2), i = key & 3;
+                if (comp4(s_v[s], i) != v) {
+                    const int ni = comp4(s_n[s], i);
+                    if (ni < 0) hh = true;
+                    else {
+                        valid = true;
+                        cand = ni;
+                    }
+                }
+            }
+            if (__ballot(hh)) {
+                hull = true;
+                break;
+            }
+            unsigned long long pend = __ballot(valid);
+            while (pend) {
+                const int leader = __ffsll((unsigned long long)pend) - 1;
+                const int val = __builtin_amdgcn_readlane(cand, leader);
+                const unsigned long long same = __ballot(valid && cand == val);
+                bool found = false;
+#pragma unroll
+                for (int k = 0; k < NS; ++k)
+                    if (__ballot(sid[k] == val)) found = true;
+                if (!found) {
+                    if (S < SMAX) {
+#pragma unroll
+                        for (int k = 0; k < NS; ++k)
+                            if (k == (S >> 6) && lane == (S & 63)) sid[k] = val;
+                    }
+                    S++;
+                }
+                pend &= ~same;
+            }
+        }
+        if (hull) break;
+        if (S > SMAX) {
+            over = true;
+            break;
+        }
+        for (int b = f1; b < S; b += 64) {
+            const int s = b + lane;
+            int id = -1;
+#pragma unroll
+            for (int k = 0; k < NS; ++k) {
+                const int t = __shfl(sid[k], s & 63);
+                if ((s >> 6) == k) id = t;
+            }
+            if (s < S) {
+                s_v[s] = in.tets[2 * id];
+                s_n[s] = in.tets[2 * id + 1];
+            }
+        }
+        __syncthreads();
+        f0 = f1;
+        f1 = S;
+    }
+    if (hull) {
+        if (lane == 0) status[v] = EX_INCOMPLETE;
+        return;
+    }
+    if (over) {
+        if (lane == 0) {
+            status[v] = -1;
+            atomicOr(&ctrs[C_ERR], 4u);
+        }
+        return;
+    }
+    // thresholds as the reference computes them (float)
+    float lo[3], hi[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        const float e = (gp.dmax[i] - gp.dmin[i]) * 2.0f * FLT_EPSILON;
+        lo[i] = gp.dmin[i] - e;
+        hi[i] = gp.dmax[i] + e;
+    }
+    bool decided = false;
+    int st = EX_DEPOSIT;
+    if (prec == 0) {
+        // FP32 circumcenters with their error bounds (cc.w)
+        float mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
+        float err = 0.0f;
+        for (int k = 0; k < NS; ++k) {
+            const int s = k * 64 + lane;
+            if (s < S) {
+                const float4 c = in.cc[sid[k]];
+                mn[0] = fminf(mn[0], c.x);
+                mn[1] = fminf(mn[1], c.y);
+                mn[2] = fminf(mn[2], c.z);
+                mx[0] = fmaxf(mx[0], c.x);
+                mx[1] = fmaxf(mx[1], c.y);
+                mx[2] = fmaxf(mx[2], c.z);
+                // NaN or infinite bound: undecidable in float
+                err = (c.w >= 0.0f) ? fmaxf(err, c.w) : INFINITY;
+            }
+        }
+        for (int off = 32; off >= 1; off >>= 1) {
+#pragma unroll
+            for (int i = 0; i < 3; ++i) {
+                mn[i] = fminf(mn[i], __shfl_xor(mn[i], off));
+                mx[i] = fmaxf(mx[i], __shfl_xor(mx[i], off));
+            }
+            err = fmaxf(err, __shfl_xor(err, off));
+        }
+        if (err < 1e-3f) {
+            bool surely_in = true, surely_out = false;
+#pragma unroll
+            for (int i = 0; i < 3; ++i) {
+                surely_in &= (mn[i] - err > lo[i]) && (mx[i] + err < hi[i]);
+                surely_out |= (mn[i] + err < lo[i]) || (mx[i] - err > hi[i]);
+            }
+            if (surely_in || surely_out) {
+                decided = true;
+                st = surely_out ? EX_OUTSIDE : EX_DEPOSIT;
+            }
+        }
+        if (!decided && lane == 0) atomicAdd(&ctrs[C_FALLBACK], 1u);
+    }
+    if (!decided) {
+        // flat tets have no circumcenter (NaN) and are skipped
+        double mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
+        bool defined = false;
+        for (int k = 0; k < NS; ++k) {
+            const int s = k * 64 + lane;
+            if (s < S) {
+                const int4 vv = s_v[s];
+                const float4 a = in.pts[vv.x], b = in.pts[vv.y], c = in.pts[vv.z], d = in.pts[vv.w];
+                double ox, oy, oz;
+                circumcenter_f64(a.x, a.y, a.z, b.x, b.y, b.z, c.x, c.y, c.z, d.x, d.y, d.z, ox, oy,
+                                 oz);
+                if (isnan(ox) || isnan(oy) || isnan(oz)) continue;
+                defined = true;
+                mn[0] = fmin(mn[0], ox);
+                mn[1] = fmin(mn[1], oy);
+                mn[2] = fmin(mn[2], oz);
+                mx[0] = fmax(mx[0], ox);
+                mx[1] = fmax(mx[1], oy);
+                mx[2] = fmax(mx[2], oz);
+            }
+        }
+        for (int off = 32; off >= 1; off >>= 1) {
+#pragma unroll
+            for (int i = 0; i < 3; ++i) {
+                mn[i] = fmin(mn[i], __shfl_xor(mn[i], off));
+                mx[i] = fmax(mx[i], __shfl_xor(mx[i], off));
+            }
+        }
+        const bool anydef = __ballot(defined) != 0ull;
+        if (!anydef) st = EX_DEGENERATE;
+        else if (mn[0] < (double)lo[0] || mn[1] < (double)lo[1] || mn[2] < (double)lo[2] ||
+                 mx[0] > (double)hi[0] || mx[1] > (double)hi[1] || mx[2] > (double)hi[2])
+            st = EX_OUTSIDE;
+        else st = EX_DEPOSIT;
+    }
+    if (lane == 0) status[v] = st;
+}
+
+void launch_exact_status(const BlockIn &in, const GridP &gp, int *status, int prec, unsigned *ctrs,
+                         hipStream_t s)
+{
+    if (in.ncells <= 0) return;
+    hipLaunchKernelGGL((k_exact_status<8>), dim3(in.ncells), dim3(64), 0, s, in, gp, status, prec,
+                       ctrs);
+}
+
+// ---------------------------------------------------------------------------
+// counts, density, CIC of deposited cells without points
+// ---------------------------------------------------------------------------
+__global__ void k_exact_count(const int *__restrict__ owner, size_t n, const int *__restrict__ status,
+                              int ncells, int *__restrict__ npts)
+{
+    for (size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x; g < n;
+         g += (size_t)gridDim.x * blockDim.x) {
+        const int o = owner[g];
+        if (o >= 0 && o < ncells && status[o] == EX_DEPOSIT) atomicAdd(&npts[o], 1);
+    }
+}
+
+__global__ void k_exact_density(GridP gp, const int *__restrict__ owner, const int *__restrict__ status,
+                                const int *__restrict__ npts, int ncells, CellOut o, int prec)
+{
+    const size_t n = (size_t)gp.bnum_idx[0] * gp.bnum_idx[1] * gp.bnum_idx[2];
+    for (size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x; g < n;
+         g += (size_t)gridDim.x * blockDim.x) {
+        const int ow = owner[g];
+        if (ow < 0 || ow >= ncells || status[ow] != EX_DEPOSIT) continue;
+        const int lx = (int)(g % gp.bnum_idx[0]);
+        const int ly = (int)((g / gp.bnum_idx[0]) % gp.bnum_idx[1]);
+        const int lz = (int)(g / ((size_t)gp.bnum_idx[0] * gp.bnum_idx[1]));
+        const int N = npts[ow];
+        const float m = prec == 0 ? gp.mass / (float)N : (float)((double)gp.mass / (double)N);
+        deposit_point(gp.bmin_idx[0] + lx, gp.bmin_idx[1] + ly, gp.bmin_idx[2] + lz, m, ow, gp,
+                         o);
+    }
+}
+
+__global__ void k_exact_cic(BlockIn in, GridP gp, const int *__restrict__ status_ex,
+                            CellOut o)
+{
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= in.ncells) return;
+    const int s = status_ex[v];
+    int out_status = s, out_n = 0;
+    if (s == EX_DEPOSIT) {
+        const int N = o.npts[v];
+        if (N > 0) {
+            out_status = 0;
+            out_n = N;
+        } else {
+            const float4 p = in.pts[v];
+            int mn[3];
+            float m8[8];
+            cic_weights(p.x, p.y, p.z, gp.mass, gp, mn, m8);
+            for (int k = 0; k < 8; ++k)
+                deposit_point(mn[0] + (k & 1), mn[1] + ((k >> 1) & 1), mn[2] + (k >> 2), m8[k], v,
+                                 gp, o);
+            out_status = 1;
+            out_n = 8;
+        }
+    }
+    o.status[v] = out_status;
+    if (s != EX_DEPOSIT) o.npts[v] = 0;
+    else o.npts[v] = out_n;
+}
+
+void launch_exact_tail(const BlockIn &in, const GridP &gp, const int *owner, const int *status_ex,
+                       const CellOut &o, int prec, hipStream_t s)
+{
+    const size_t n = (size_t)gp.bnum_idx[0] * gp.bnum_idx[1] * gp.bnum_idx[2];
+    size_t blocks = (n + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    if (blocks == 0) blocks = 1;
+    if (in.ncells > 0) launch_fill_u32((unsigned *)o.npts, (size_t)in.ncells, 0u, s);
+    hipLaunchKernelGGL(k_exact_count, dim3((unsigned)blocks), dim3(256), 0, s, owner, n, status_ex,
+                       in.ncells, o.npts);
+    hipLaunchKernelGGL(k_exact_density, dim3((unsigned)blocks), dim3(256), 0, s, gp, owner, status_ex,
+                       o.npts, in.ncells, o, prec);
+    if (in.ncells > 0)
+        hipLaunchKernelGGL(k_exact_cic, dim3((in.ncells + 255) / 256), dim3(256), 0, s, in, gp,
+                           status_ex, o);
+}
+
+}  // namespace t2

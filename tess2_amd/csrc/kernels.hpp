@@ -50,6 +50,15 @@ struct CellOut {
     unsigned slow_cap;
 };
 
+// uniform bucket hash of the particles: bucket (i,j,k) holds the particles
+// with floor((x - lo) * inv) == (i,j,k), clamped to the grid of dims buckets
+struct HashP {
+    float lo[3];
+    float inv;      // 1 / cell
+    float cell;
+    int dims[3];
+};
+
 struct BlockIn {
     const float4 *pts;   // xyz + pad
     const int4 *tets;    // 2 int4 per tet: verts, neighbours
@@ -59,8 +68,17 @@ struct BlockIn {
     int ntets;
 };
 
+// circumcenters; cc.w = error bound of the float solve (inf when undefined)
 void launch_circumcenters(const float4 *pts, const int4 *tets, int ntets, float4 *cc,
                           hipStream_t s);
+void launch_hash_build(const float4 *pts, int np, const HashP &h, unsigned *start,
+                       unsigned *fill, unsigned *sums, int *bk, float4 *sorted, hipStream_t s);
+void launch_owner(const GridP &gp, const HashP &h, const unsigned *start, const float4 *hp,
+                  int *owner, int prec, unsigned *ctrs, hipStream_t s);
+void launch_exact_status(const BlockIn &in, const GridP &gp, int *status, int prec,
+                         unsigned *ctrs, hipStream_t s);
+void launch_exact_tail(const BlockIn &in, const GridP &gp, const int *owner,
+                       const int *status_ex, const CellOut &o, int prec, hipStream_t s);
 void launch_fill_u32(unsigned *p, size_t n, unsigned v, hipStream_t s);
 void launch_cells_ref(const BlockIn &in, const GridP &gp, const CellOut &o,
                       int want_density, hipStream_t s);

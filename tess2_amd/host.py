@@ -210,6 +210,11 @@ class StagedRun:
         out["density"], out["volumes"], out["circumcenters"] = d, v, c
         return out
 
+    def owner(self, b, ngrid):
+        o = np.zeros(ngrid, np.int32)
+        self.ctx.check(N.lib().tess2_fetch_owner(self.ctx.h, int(b), o.ctypes.data), "owner")
+        return o
+
     def cells(self, b):
         hb = self.blocks[b]
         s = np.zeros(hb.num_orig, np.int32)

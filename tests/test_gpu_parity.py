@@ -20,16 +20,23 @@ def say(*a):
     sys.stdout.flush()
 
 
+def _canon(a):
+    # every NaN compares equal (x86 and the GPU produce different NaN payloads)
+    u = np.ascontiguousarray(a, np.float32).ravel().view(np.uint32).copy()
+    u[np.isnan(np.ascontiguousarray(a, np.float32).ravel())] = 0x7FC00000
+    return u
+
+
 def bits_equal(a, b):
     a = np.ascontiguousarray(a, np.float32)
     b = np.ascontiguousarray(b, np.float32)
-    return a.shape == b.shape and np.array_equal(a.view(np.uint32), b.view(np.uint32))
+    return a.shape == b.shape and np.array_equal(_canon(a), _canon(b))
 
 
 def first_diff(a, b):
     a = np.ascontiguousarray(a, np.float32).ravel()
     b = np.ascontiguousarray(b, np.float32).ravel()
-    d = np.nonzero(a.view(np.uint32) != b.view(np.uint32))[0]
+    d = np.nonzero(_canon(a) != _canon(b))[0]
     return (len(d), d[:5].tolist(), a[d[:5]].tolist(), b[d[:5]].tolist())
 
 
@@ -41,10 +48,12 @@ def blocks():
     p = rng.random((4096, 3)).astype(np.float32)
     out["p2"] = ingest.make_block(p, 4096, (0, 0, 0), (1, 1, 1))
     out["grf"] = ingest.periodic_block(synthetic.grf_lattice(24, 0.3, seed=5), 4.0 / 24)
+    # exact lattice: Qhull's Qt output has flat tets with NaN circumcenters
+    out["lattice"] = ingest.periodic_block(synthetic.jittered_lattice(8, 0.0, seed=4), 3.0 / 8)
     return out
 
 
-@pytest.mark.parametrize("name", ["periodic4k", "p2", "grf"])
+@pytest.mark.parametrize("name", ["periodic4k", "p2", "grf", "lattice"])
 def test_circumcenters_bitexact(oracle, blocks, name):
     hb = blocks[name]
     say("circumcenters", name, hb.num_tets)
@@ -53,7 +62,7 @@ def test_circumcenters_bitexact(oracle, blocks, name):
     assert bits_equal(got, want), first_diff(got, want)
 
 
-@pytest.mark.parametrize("name", ["periodic4k", "p2", "grf"])
+@pytest.mark.parametrize("name", ["periodic4k", "p2", "grf", "lattice"])
 def test_volumes_bitexact(oracle, blocks, name):
     hb = blocks[name]
     say("volumes", name, hb.num_orig)
@@ -68,7 +77,8 @@ def _oracle_dense(oracle, hb, g, cic=False, mass=1.0):
 
 
 @pytest.mark.parametrize("name,g", [("periodic4k", 32), ("periodic4k", 64), ("p2", 64),
-                                    ("periodic4k", 8), ("grf", 40)])
+                                    ("periodic4k", 8), ("grf", 40), ("lattice", 17),
+                                    ("lattice", 33)])
 def test_dense_reference_scan_bitexact(oracle, blocks, name, g):
     hb = blocks[name]
     say("dense", name, g)

@@ -220,3 +220,29 @@ def test_exact_owner_brute_force(oracle):
     brute = np.argmin(d2, axis=1)   # first minimum = lowest id
     assert np.array_equal(own, brute)
     assert own[(4 * 9 + 4) * 9 + 4] == 5
+
+
+def test_exact_dense_oracle_invariants(oracle):
+    """Exact-Voronoi FP64 oracle: every owner is the brute-force nearest site,
+    deposited mass is conserved, and each deposited cell with points owns
+    exactly npts grid points."""
+    p = synthetic.uniform(1500, seed=23)
+    hb = ingest.periodic_block(p, 4.0 / 1500 ** (1 / 3))
+    ob = oracle.Block.from_host(hb)
+    g = 20
+    r = oracle.exact_dense(ob, (g, g, g), mass=3.0)
+    own = r["owner"]
+    step, gmin = r["grid_step_size"], r["grid_phys_mins"]
+    zz, yy, xx = np.meshgrid(*(np.arange(g),) * 3, indexing="ij")
+    gp = (np.stack([xx, yy, zz], -1).reshape(-1, 3).astype(np.float32) * step + gmin)
+    P = hb.particles.astype(np.float64)
+    pick = np.random.default_rng(0).choice(gp.shape[0], 800, replace=False)
+    d = (gp[pick, None, :].astype(np.float64) - P[None]) ** 2
+    brute = np.argmin((d[..., 0] + d[..., 1]) + d[..., 2], axis=1)
+    assert np.array_equal(own[pick], brute)
+    st, n = r["status"], r["npts"]
+    dep = (st == 0) | (st == 1)
+    for c in np.nonzero(st == 0)[0][:200]:
+        assert (own == c).sum() == n[c]
+    div = float(step[0] * step[1] * step[2])          # float product, as the reference
+    assert abs(r["density"].sum() * div - 3.0 * dep.sum()) < 1e-9 * 3.0 * dep.sum() + 1e-6

@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdint>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -26,6 +27,8 @@ struct DevBlock {
     int4 *tets = nullptr;
     int *v2t = nullptr;
     float4 *cc = nullptr;
+    int *order = nullptr;                    // cells in Morton order
+    std::vector<int> tet_new;                // caller's tet id -> device tet id
     float *vol = nullptr;
     int *status = nullptr, *npts = nullptr;
     // grid
@@ -120,7 +123,7 @@ static int dalloc(tess2_ctx *c, T **p, size_t n)
 
 static void free_block(DevBlock &b)
 {
-    void *ps[] = {b.pts,    b.tets,   b.v2t,   b.cc,   b.vol,  b.status,    b.npts,
+    void *ps[] = {b.order,  b.pts,    b.tets,   b.v2t,   b.cc,   b.vol,  b.status,    b.npts,
                   b.density, b.count, b.first, b.head, b.ovf,  b.slow,      b.ctrs,
                   b.cstat,  b.gsum,   b.gmax,  b.owner, b.status_ex, b.hstart, b.hfill,
                   b.hsums,  b.hbk,    b.hsorted};
@@ -224,6 +227,73 @@ extern "C" const char *tess2_last_error(const tess2_ctx *c)
 // ---------------------------------------------------------------------------
 // staging
 // ---------------------------------------------------------------------------
+static inline uint32_t spread3(uint32_t x)
+{
+    x &= 0x3ffu;
+    x = (x | (x << 16)) & 0x030000FFu;
+    x = (x | (x << 8)) & 0x0300F00Fu;
+    x = (x | (x << 4)) & 0x030C30C3u;
+    x = (x | (x << 2)) & 0x09249249u;
+    return x;
+}
+
+// Morton ranks of all particles, the original cells in Morton order, and the
+// tets renumbered by the smallest Morton rank among their vertices.
+static void morton_layout(const tess2_block_t &hb, std::vector<int> &rank,
+                          std::vector<int> &order, std::vector<int> &tnew,
+                          std::vector<tess2_tet_t> &tl, std::vector<int> &v2t)
+{
+    const int np = hb.num_particles, nt = hb.num_tets;
+    float lo[3] = {0, 0, 0}, hi[3] = {1, 1, 1};
+    for (int p = 0; p < np; ++p)
+        for (int i = 0; i < 3; ++i) {
+            const float x = hb.particles[3 * p + i];
+            if (p == 0 || x < lo[i]) lo[i] = x;
+            if (p == 0 || x > hi[i]) hi[i] = x;
+        }
+    std::vector<uint64_t> key(np);
+    for (int p = 0; p < np; ++p) {
+        uint32_t q[3];
+        for (int i = 0; i < 3; ++i) {
+            const double ext = (double)hi[i] - lo[i];
+            double f = ext > 0 ? ((double)hb.particles[3 * p + i] - lo[i]) / ext : 0.0;
+            q[i] = (uint32_t)std::min(1023.0, std::max(0.0, f * 1023.0));
+        }
+        const uint32_t code = spread3(q[0]) | (spread3(q[1]) << 1) | (spread3(q[2]) << 2);
+        key[p] = ((uint64_t)code << 32) | (uint32_t)p;
+    }
+    std::sort(key.begin(), key.end());
+    rank.assign(np, 0);
+    order.clear();
+    order.reserve(hb.num_orig_particles);
+    for (int k = 0; k < np; ++k) {
+        const int p = (int)(key[k] & 0xffffffffu);
+        rank[p] = k;
+        if (p < hb.num_orig_particles) order.push_back(p);
+    }
+    std::vector<uint64_t> tk(nt);
+    for (int t = 0; t < nt; ++t) {
+        int m = rank[hb.tets[t].verts[0]];
+        for (int i = 1; i < 4; ++i) m = std::min(m, rank[hb.tets[t].verts[i]]);
+        tk[t] = ((uint64_t)(uint32_t)m << 32) | (uint32_t)t;
+    }
+    std::sort(tk.begin(), tk.end());
+    tnew.assign(nt, 0);
+    for (int k = 0; k < nt; ++k) tnew[(int)(tk[k] & 0xffffffffu)] = k;
+    tl.resize(nt);
+    for (int k = 0; k < nt; ++k) {
+        const tess2_tet_t &q = hb.tets[(int)(tk[k] & 0xffffffffu)];
+        tess2_tet_t &o = tl[k];
+        for (int i = 0; i < 4; ++i) {
+            o.verts[i] = q.verts[i];
+            o.tets[i] = q.tets[i] < 0 ? q.tets[i] : tnew[q.tets[i]];
+        }
+    }
+    v2t.resize(np);
+    for (int p = 0; p < np; ++p)
+        v2t[p] = hb.vert_to_tet[p] < 0 ? hb.vert_to_tet[p] : tnew[hb.vert_to_tet[p]];
+}
+
 static int validate(tess2_ctx *c, const tess2_block_t &b)
 {
     if (b.num_orig_particles < 0 || b.num_particles < b.num_orig_particles || b.num_tets < 0)
@@ -285,9 +355,23 @@ extern "C" int tess2_stage(tess2_ctx *c, const tess2_block_t *blocks, int nblock
             tmp[p] = make_float4(hb.particles[3 * p], hb.particles[3 * p + 1],
                                  hb.particles[3 * p + 2], 0.0f);
         HIPCHK(c, hipMemcpy(d.pts, tmp.data(), sizeof(float4) * d.np, hipMemcpyHostToDevice));
-        HIPCHK(c, hipMemcpy(d.tets, hb.tets, sizeof(tess2_tet_t) * (size_t)d.nt,
+        // HBM layout: cells visited in Morton order of their sites; tets
+        // renumbered by the Morton rank of their first-ranked vertex so that
+        // the stars of cells processed together are close in memory.  The
+        // results do not depend on either order (the star traversal follows
+        // neighbour slots, not tet ids).
+        std::vector<int> rank, order, tnew;
+        std::vector<tess2_tet_t> tl;
+        std::vector<int> v2t;
+        morton_layout(hb, rank, order, tnew, tl, v2t);
+        d.tet_new.swap(tnew);
+        if ((r = dalloc(c, &d.order, (size_t)d.norig))) return r;
+        if (d.norig)
+            HIPCHK(c, hipMemcpy(d.order, order.data(), sizeof(int) * (size_t)d.norig,
+                                hipMemcpyHostToDevice));
+        HIPCHK(c, hipMemcpy(d.tets, tl.data(), sizeof(tess2_tet_t) * (size_t)d.nt,
                             hipMemcpyHostToDevice));
-        HIPCHK(c, hipMemcpy(d.v2t, hb.vert_to_tet, sizeof(int) * (size_t)d.np,
+        HIPCHK(c, hipMemcpy(d.v2t, v2t.data(), sizeof(int) * (size_t)d.np,
                             hipMemcpyHostToDevice));
         d.slow_cap = (unsigned)std::max(1024, d.norig / 32);
         if ((r = dalloc(c, &d.slow, d.slow_cap))) return r;
@@ -371,6 +455,7 @@ static BlockIn make_in(DevBlock &d)
     in.tets = d.tets;
     in.v2t = d.v2t;
     in.cc = d.cc;
+    in.order = d.order;
     in.ncells = d.norig;
     in.ntets = d.nt;
     return in;
@@ -600,9 +685,10 @@ extern "C" int tess2_fetch(tess2_ctx *c, int b, float *density, float *volumes,
         std::vector<float4> tmp(d.nt);
         HIPCHK(c, hipMemcpy(tmp.data(), d.cc, sizeof(float4) * d.nt, hipMemcpyDeviceToHost));
         for (int t = 0; t < d.nt; ++t) {
-            circumcenters[3 * t] = tmp[t].x;
-            circumcenters[3 * t + 1] = tmp[t].y;
-            circumcenters[3 * t + 2] = tmp[t].z;
+            const float4 q = tmp[d.tet_new[t]];   // caller's numbering
+            circumcenters[3 * t] = q.x;
+            circumcenters[3 * t + 1] = q.y;
+            circumcenters[3 * t + 2] = q.z;
         }
     }
     return 0;

@@ -534,8 +534,9 @@ __global__ __launch_bounds__(64) void k_cells_ref(BlockIn in, GridP gp, CellOut 
     __shared__ int s_ns[64 * NS * 4];
     __shared__ float4 s_cc[64 * NS];
     __shared__ int4 s_u[(64 * NS > RMAX) ? 64 * NS : RMAX];
-    const int v = blockIdx.x;
-    if (v >= in.ncells) return;
+    const int k = xcd_item((int)blockIdx.x, in.ncells);
+    if (k < 0) return;
+    const int v = in.order ? in.order[k] : k;
     cell_ref_body<NS, NF, RMAX>(v, in, gp, o, want_density, true, s_v, s_ns, s_cc, s_u);
 }
 
@@ -562,8 +563,8 @@ void launch_cells_ref(const BlockIn &in, const GridP &gp, const CellOut &o, int 
                       hipStream_t s)
 {
     if (in.ncells <= 0) return;
-    hipLaunchKernelGGL((k_cells_ref<2, 1, 128>), dim3(in.ncells), dim3(64), 0, s, in, gp, o,
-                       want_density);
+    hipLaunchKernelGGL((k_cells_ref<1, 1, 64>), dim3(xcd_grid(in.ncells)), dim3(64), 0, s, in,
+                       gp, o, want_density);
 }
 
 void launch_cells_ref_slow(const BlockIn &in, const GridP &gp, const CellOut &o,

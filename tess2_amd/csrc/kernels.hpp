@@ -64,9 +64,22 @@ struct BlockIn {
     const int4 *tets;    // 2 int4 per tet: verts, neighbours
     const int *v2t;
     const float4 *cc;    // circumcenters (xyz + pad)
+    const int *order;    // cells in processing (Morton) order, or nullptr
     int ncells;          // original particles
     int ntets;
 };
+
+// Workgroup -> work item with XCD affinity: the dispatcher hands consecutive
+// workgroups to the 8 XCDs in turn, so item k = (b % 8) * chunk + b / 8 gives
+// each XCD (and its L2) one contiguous range of items.  Returns -1 for the
+// padding workgroups of the last range.
+__device__ __forceinline__ int xcd_item(int b, int n)
+{
+    const int chunk = (n + 7) >> 3;
+    const int k = (b & 7) * chunk + (b >> 3);
+    return k < n ? k : -1;
+}
+__host__ __forceinline__ int xcd_grid(int n) { return ((n + 7) >> 3) << 3; }
 
 // circumcenters; cc.w = error bound of the float solve (inf when undefined)
 void launch_circumcenters(const float4 *pts, const int4 *tets, int ntets, float4 *cc,

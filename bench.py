@@ -90,6 +90,41 @@ def cpu_baseline(hb, grid, target_s):
             "est_full_s": est}
 
 
+def exact_paths(run, grid, G, args):
+    """The exact-Voronoi assignment in the mixed FP32/FP64 path and in the
+    all-FP64 path on the same staged block: timing, FP64 fallbacks, and the
+    agreement of the two (owner of every grid point, densities)."""
+    from tess2_amd import host
+    res = {}
+    outs = {}
+    for name, prec in (("mixed", host.MIXED), ("fp64", host.FP64)):
+        for _ in range(args.warmup):
+            run.run(grid, assignment=host.EXACT_VORONOI, precision=prec, want_volumes=False)
+            run.sync()
+        run.kernel_time()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            run.run(grid, assignment=host.EXACT_VORONOI, precision=prec, want_volumes=False)
+            st = run.sync()
+        ms = (time.perf_counter() - t0) * 1e3 / args.steps
+        kms, _ = run.kernel_time()
+        outs[name] = (run.owner(0, G), run.fetch(0, ngrid=G)["density"])
+        res[name] = {"ms_per_step": ms, "grid_pts_per_s": G / (ms * 1e-3),
+                     "owner_kernel_ms": kms, "fp64_fallbacks": int(st["fp64_fallbacks"]),
+                     "cells_deposited": int(st["cells_scan"] + st["cells_cic"])}
+        log(f"[exact {name}] {ms:.3f} ms/step, owner kernel {kms:.3f} ms, "
+            f"fallbacks {st['fp64_fallbacks']}")
+    om, dm = outs["mixed"]
+    of, df = outs["fp64"]
+    nz = df != 0
+    rel = np.abs(dm[nz].astype(np.float64) - df[nz]) / np.abs(df[nz].astype(np.float64))
+    res["speedup_mixed_vs_fp64"] = res["fp64"]["ms_per_step"] / res["mixed"]["ms_per_step"]
+    res["owner_identical"] = bool(np.array_equal(om, of))
+    res["max_rel_density_diff"] = float(rel.max()) if rel.size else 0.0
+    res["rms_rel_density_diff"] = float(np.sqrt(np.mean(rel ** 2))) if rel.size else 0.0
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -97,6 +132,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--config", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-exact", action="store_true",
+                    help="skip the exact-Voronoi mixed / FP64 measurement")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--save-block", default=None,
                     help="tessellate, write the block to this .npz and exit")
@@ -161,6 +198,8 @@ def main():
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "k_cells_ref", "kernel_ms": kern_ms, "alg_bytes": b_alg},
     }
+    if not args.no_exact:
+        out["exact_voronoi"] = exact_paths(run, grid, G, args)
     if not args.no_cpu:
         log("[cpu] timing the oracle on a sample")
         out["cpu_baseline"] = cpu_baseline(hb, grid, args.cpu_seconds)

@@ -527,9 +527,10 @@ static int enqueue_run(tess2_ctx *c, const RunArgs &a)
                 launch_cells_ref_slow(in, gp, ov, 0, s);
             }
             launch_hash_build(d.pts, d.np, d.hash, d.hstart, d.hfill, d.hsums, d.hbk, d.hsorted, s);
-            launch_exact_status(in, gp, d.status_ex, a.precision, d.ctrs, s);
+            launch_exact_status(in, gp, d.status_ex, a.precision, d.ctrs, d.slow, d.slow_cap, s);
             if (first_block) HIPCHK(c, hipEventRecord(c->ev_k0, s));
-            launch_owner(gp, d.hash, d.hstart, d.hsorted, d.owner, a.precision, d.ctrs, s);
+            launch_owner(gp, d.hash, d.hstart, d.hsorted, d.owner, a.precision, d.ctrs,
+                         d.status_ex, d.npts, d.norig, s);
             if (first_block) HIPCHK(c, hipEventRecord(c->ev_k1, s));
             launch_exact_tail(in, gp, d.owner, d.status_ex, o, a.precision, s);
         } else if (a.alg == TESS2_DENSE_CIC) {

@@ -252,42 +252,59 @@ __global__ __launch_bounds__(256) void k_owner(GridP gp, HashP h,
                                                const unsigned *__restrict__ start,
                                                const float4 *__restrict__ hp, int *__restrict__ owner,
                                                int prec, unsigned *ctrs, int tiles_x, int tiles_y,
-                                               int ntiles)
+                                               int ntiles, const int *__restrict__ status,
+                                               int *__restrict__ npts, int ncells)
 {
     const int wave = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (wave >= ntiles) return;
     const int lane = threadIdx.x & 63;
     const int tx = wave % tiles_x, ty = (wave / tiles_x) % tiles_y, tz = wave / (tiles_x * tiles_y);
-    const int lx = tx * 4 + (lane & 3), ly = ty * 4 + ((lane >> 2) & 3), lz = tz * 4 + (lane >> 4);
-    if (lx >= gp.bnum_idx[0] || ly >= gp.bnum_idx[1] || lz >= gp.bnum_idx[2]) return;
+    const int lx0 = tx * 4 + (lane & 3), ly0 = ty * 4 + ((lane >> 2) & 3), lz0 = tz * 4 + (lane >> 4);
+    const bool on_grid = lx0 < gp.bnum_idx[0] && ly0 < gp.bnum_idx[1] && lz0 < gp.bnum_idx[2];
+    // off-grid lanes of an edge tile stand in for an on-grid point of the
+    // tile but write nothing and count nothing
+    const int lx = on_grid ? lx0 : tx * 4, ly = on_grid ? ly0 : ty * 4, lz = on_grid ? lz0 : tz * 4;
     const long long lidx = ((long long)lz * gp.bnum_idx[1] + ly) * gp.bnum_idx[0] + lx;
     const int gx = gp.bmin_idx[0] + lx, gy = gp.bmin_idx[1] + ly, gz = gp.bmin_idx[2] + lz;
     const float px = idx2phys1(gx, gp.step[0], gp.gmin[0]);
     const float py = idx2phys1(gy, gp.step[1], gp.gmin[1]);
     const float pz = idx2phys1(gz, gp.step[2], gp.gmin[2]);
-    if (!(px >= gp.bmin[0] && px <= gp.bmax[0] && py >= gp.bmin[1] && py <= gp.bmax[1] &&
-          pz >= gp.bmin[2] && pz <= gp.bmax[2])) {
-        owner[lidx] = -1;
-        return;
-    }
+    const bool in_block = px >= gp.bmin[0] && px <= gp.bmax[0] && py >= gp.bmin[1] &&
+                          py <= gp.bmax[1] && pz >= gp.bmin[2] && pz <= gp.bmax[2];
+    if (!in_block && on_grid) owner[lidx] = -1;
     int o = -1;
-    if (prec == 0) o = nearest_f32(px, py, pz, h, start, hp);
-    if (o < 0) {
-        if (prec == 0) atomicAdd(&ctrs[C_FALLBACK], 1u);
-        o = nearest_f64(px, py, pz, h, start, hp);
+    if (in_block && on_grid) {
+        if (prec == 0) o = nearest_f32(px, py, pz, h, start, hp);
+        if (o < 0) {
+            if (prec == 0) atomicAdd(&ctrs[C_FALLBACK], 1u);
+            o = nearest_f64(px, py, pz, h, start, hp);
+        }
+        owner[lidx] = o;
     }
-    owner[lidx] = o;
+    // points per deposited cell; the lanes of a tile mostly share owners, so
+    // one atomic per distinct owner in the wave
+    const bool cnt = on_grid && in_block && o >= 0 && o < ncells && status[o] == EX_DEPOSIT;
+    unsigned long long pend = __ballot(cnt);
+    while (pend) {
+        const int leader = __ffsll((unsigned long long)pend) - 1;
+        const int val = __builtin_amdgcn_readlane(o, leader);
+        const unsigned long long same = __ballot(cnt && o == val);
+        if (lane == leader) atomicAdd(&npts[val], __popcll(same));
+        pend &= ~same;
+    }
 }
 
 void launch_owner(const GridP &gp, const HashP &h, const unsigned *start, const float4 *hp,
-                  int *owner, int prec, unsigned *ctrs, hipStream_t s)
+                  int *owner, int prec, unsigned *ctrs, const int *status, int *npts, int ncells,
+                  hipStream_t s)
 {
+    if (ncells > 0) launch_fill_u32((unsigned *)npts, (size_t)ncells, 0u, s);
     const int tx = (gp.bnum_idx[0] + 3) / 4, ty = (gp.bnum_idx[1] + 3) / 4,
               tz = (gp.bnum_idx[2] + 3) / 4;
     const int ntiles = tx * ty * tz;
     if (ntiles <= 0) return;
     hipLaunchKernelGGL(k_owner, dim3((ntiles + 3) / 4), dim3(256), 0, s, gp, h, start, hp, owner,
-                       prec, ctrs, tx, ty, ntiles);
+                       prec, ctrs, tx, ty, ntiles, status, npts, ncells);
 }
 
 // ---------------------------------------------------------------------------
@@ -300,14 +317,12 @@ __device__ __forceinline__ int comp4(const int4 &q, int i)
 }
 
 template <int NS>
-__global__ __launch_bounds__(64) void k_exact_status(BlockIn in, GridP gp, int *status, int prec,
-                                                     unsigned *ctrs)
+__device__ void exact_status_body(const int v, const BlockIn &in, const GridP &gp, int *status,
+                                  const int prec, unsigned *ctrs, int *slow,
+                                  const unsigned slow_cap, const bool can_defer, int4 *s_v,
+                                  int4 *s_n)
 {
     constexpr int SMAX = 64 * NS;
-    __shared__ int4 s_v[SMAX];
-    __shared__ int4 s_n[SMAX];
-    const int v = blockIdx.x;
-    if (v >= in.ncells) return;
     const int lane = threadIdx.x & 63;
     const int vt = in.v2t[v];
     if (vt < 0) {
@@ -394,8 +409,14 @@ __global__ __launch_bounds__(64) void k_exact_status(BlockIn in, GridP gp, int *
     }
     if (over) {
         if (lane == 0) {
-            status[v] = -1;
-            atomicOr(&ctrs[C_ERR], 4u);
+            if (can_defer) {
+                const unsigned k = atomicAdd(&ctrs[C_SLOW], 1u);
+                if (k < slow_cap) slow[k] = v;
+                else atomicOr(&ctrs[C_ERR], 1u);
+            } else {
+                status[v] = -1;
+                atomicOr(&ctrs[C_ERR], 4u);
+            }
         }
         return;
     }
@@ -488,12 +509,41 @@ __global__ __launch_bounds__(64) void k_exact_status(BlockIn in, GridP gp, int *
     if (lane == 0) status[v] = st;
 }
 
+__global__ __launch_bounds__(64) void k_exact_status(BlockIn in, GridP gp, int *status, int prec,
+                                                     unsigned *ctrs, int *slow, unsigned slow_cap)
+{
+    __shared__ int4 s_v[64];
+    __shared__ int4 s_n[64];
+    const int k = xcd_item((int)blockIdx.x, in.ncells);
+    if (k < 0) return;
+    const int v = in.order ? in.order[k] : k;
+    exact_status_body<1>(v, in, gp, status, prec, ctrs, slow, slow_cap, true, s_v, s_n);
+}
+
+__global__ __launch_bounds__(64) void k_exact_status_slow(BlockIn in, GridP gp, int *status,
+                                                          int prec, unsigned *ctrs, int *slow,
+                                                          unsigned slow_cap)
+{
+    __shared__ int4 s_v[512];
+    __shared__ int4 s_n[512];
+    unsigned n = ctrs[C_SLOW];
+    if (n > slow_cap) n = slow_cap;
+    for (unsigned k = blockIdx.x; k < n; k += gridDim.x) {
+        exact_status_body<8>(slow[k], in, gp, status, prec, ctrs, slow, slow_cap, false, s_v,
+                             s_n);
+        __syncthreads();
+    }
+}
+
 void launch_exact_status(const BlockIn &in, const GridP &gp, int *status, int prec, unsigned *ctrs,
-                         hipStream_t s)
+                         int *slow, unsigned slow_cap, hipStream_t s)
 {
     if (in.ncells <= 0) return;
-    hipLaunchKernelGGL((k_exact_status<8>), dim3(in.ncells), dim3(64), 0, s, in, gp, status, prec,
-                       ctrs);
+    launch_fill_u32(&ctrs[C_SLOW], 1, 0u, s);   // the list is shared with the volume pass
+    hipLaunchKernelGGL(k_exact_status, dim3(xcd_grid(in.ncells)), dim3(64), 0, s, in, gp, status,
+                       prec, ctrs, slow, slow_cap);
+    hipLaunchKernelGGL(k_exact_status_slow, dim3(256), dim3(64), 0, s, in, gp, status, prec, ctrs,
+                       slow, slow_cap);
 }
 
 // ---------------------------------------------------------------------------
@@ -563,9 +613,6 @@ void launch_exact_tail(const BlockIn &in, const GridP &gp, const int *owner, con
     size_t blocks = (n + 255) / 256;
     if (blocks > 65536) blocks = 65536;
     if (blocks == 0) blocks = 1;
-    if (in.ncells > 0) launch_fill_u32((unsigned *)o.npts, (size_t)in.ncells, 0u, s);
-    hipLaunchKernelGGL(k_exact_count, dim3((unsigned)blocks), dim3(256), 0, s, owner, n, status_ex,
-                       in.ncells, o.npts);
     hipLaunchKernelGGL(k_exact_density, dim3((unsigned)blocks), dim3(256), 0, s, gp, owner, status_ex,
                        o.npts, in.ncells, o, prec);
     if (in.ncells > 0)

@@ -87,9 +87,10 @@ void launch_circumcenters(const float4 *pts, const int4 *tets, int ntets, float4
 void launch_hash_build(const float4 *pts, int np, const HashP &h, unsigned *start,
                        unsigned *fill, unsigned *sums, int *bk, float4 *sorted, hipStream_t s);
 void launch_owner(const GridP &gp, const HashP &h, const unsigned *start, const float4 *hp,
-                  int *owner, int prec, unsigned *ctrs, hipStream_t s);
+                  int *owner, int prec, unsigned *ctrs, const int *status, int *npts,
+                  int ncells, hipStream_t s);
 void launch_exact_status(const BlockIn &in, const GridP &gp, int *status, int prec,
-                         unsigned *ctrs, hipStream_t s);
+                         unsigned *ctrs, int *slow, unsigned slow_cap, hipStream_t s);
 void launch_exact_tail(const BlockIn &in, const GridP &gp, const int *owner,
                        const int *status_ex, const CellOut &o, int prec, hipStream_t s);
 void launch_fill_u32(unsigned *p, size_t n, unsigned v, hipStream_t s);

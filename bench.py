@@ -7,7 +7,11 @@ the GPU per step: circumcenters -> stars -> volumes -> face planes ->
 scanline assignment -> deposit onto a 256^3 float32 grid, all with inputs
 resident in HBM.  One JSON line on stdout.
 
-  python bench.py [--steps K] [--warmup W] [--config {0,1}] [--no-cpu]
+With --config 2: 64 Plummer-cluster blocks of uneven size (5k..200k
+particles, log-uniform), each with its ghost layer, 64^3 grid per block, all
+blocks in one persistent launch (work split chosen by the autotuner).
+
+  python bench.py [--steps K] [--warmup W] [--config {0,1,2}] [--no-cpu]
 """
 import argparse
 import json
@@ -29,29 +33,95 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def _save_blocks(path, blocks):
+    z = {}
+    for i, hb in enumerate(blocks):
+        for k in ("particles", "tets", "vert_to_tet", "bounds_min", "bounds_max"):
+            z[f"b{i}_{k}"] = getattr(hb, k)
+        z[f"b{i}_num_orig"] = np.int64(hb.num_orig)
+        z[f"b{i}_gid"] = np.int64(hb.gid)
+    np.savez(path, nblocks=np.int64(len(blocks)), **z)
+
+
+def _load_blocks(path):
+    from tess2_amd import ingest
+    z = np.load(path)
+    return [ingest.HostBlock(particles=z[f"b{i}_particles"], tets=z[f"b{i}_tets"],
+                             vert_to_tet=z[f"b{i}_vert_to_tet"],
+                             num_orig=int(z[f"b{i}_num_orig"]),
+                             bounds_min=z[f"b{i}_bounds_min"], bounds_max=z[f"b{i}_bounds_max"],
+                             gid=int(z[f"b{i}_gid"]))
+            for i in range(int(z["nblocks"]))]
+
+
 def setup(cfg, load=None):
+    """-> (workload description, list of blocks)"""
     from tess2_amd import ingest, synthetic
     c = synthetic.CONFIGS[cfg]
     if load:
-        z = np.load(load)
-        hb = ingest.HostBlock(particles=z["particles"], tets=z["tets"],
-                              vert_to_tet=z["vert_to_tet"], num_orig=int(z["num_orig"]),
-                              bounds_min=z["bounds_min"], bounds_max=z["bounds_max"])
-        log(f"[setup] loaded {load}: {hb.num_orig} particles, {hb.num_tets} tets")
-        return c, hb
+        blocks = _load_blocks(load)
+        log(f"[setup] loaded {load}: {len(blocks)} blocks, "
+            f"{sum(b.num_orig for b in blocks)} particles, {sum(b.num_tets for b in blocks)} tets")
+        return c, blocks
     t0 = time.time()
+    workers = ingest.default_workers()
+    if c["kind"] == "plummer":
+        ns = c["nblocks_side"]
+        pts, bid, counts, bounds = synthetic.plummer_blocks(ns, c["nmin"], c["nmax"], seed=cfg)
+        log(f"[setup] config {cfg} ({c['name']}): {pts.shape[0]} particles in {ns ** 3} blocks "
+            f"({counts.min()}..{counts.max()} per block)")
+        blocks = ingest.decomposed_blocks(pts, bid, bounds, c["ghost_blocks"] / ns,
+                                          workers=workers)
+        log(f"[setup] Delaunay of {len(blocks)} blocks: {sum(b.num_tets for b in blocks)} tets "
+            f"with {workers} workers ({time.time() - t0:.1f}s)")
+        return c, blocks
     p = synthetic.config_particles(cfg)
     g, src = ingest.periodic_ghosts(p, c["ghost"] / c["n_side"])
     allp = np.concatenate([p, g]).astype(np.float32)
     log(f"[setup] config {cfg} ({c['name']}): {p.shape[0]} particles + {g.shape[0]} ghosts "
         f"({time.time() - t0:.1f}s)")
-    workers = ingest.default_workers()
     t1 = time.time()
     hb = ingest.make_block(allp, p.shape[0], (0, 0, 0), (1, 1, 1), workers=workers,
                            ghost_src=src, log=log, closed_first=p.shape[0])
     log(f"[setup] Delaunay: {hb.num_tets} tets with {workers} workers "
         f"({time.time() - t1:.1f}s)")
-    return c, hb
+    return c, [hb]
+
+
+def cpu_baseline_blocks(blocks, grid, target_s):
+    """The oracle over whole blocks of the decomposition, in a seeded random
+    order, until about target_s of CPU work; scaled to all blocks by cell
+    count (cells of the sampled blocks span the same size classes)."""
+    from oracle import oracle as O
+    O.build()
+    dmin = np.min([np.asarray(b.bounds_min, np.float32) for b in blocks], axis=0)
+    dmax = np.max([np.asarray(b.bounds_max, np.float32) for b in blocks], axis=0)
+    gmin, gmax, step = O.grid_step_params(dmin, dmax, grid)
+    eps = np.float32(1e-4)
+    a = O.make_args(1.0, dmin, dmax, gmin, step, eps, grid)
+    order = np.random.default_rng(0).permutation(len(blocks))
+    spent, cells, used = 0.0, 0, 0
+    for i in order:
+        hb = blocks[i]
+        ob = O.Block.from_host(hb)
+        mn, mx, nm = O.block_grid_params(np.asarray(hb.bounds_min, np.float32),
+                                         np.asarray(hb.bounds_max, np.float32), gmin, step,
+                                         eps, dmin, dmax, grid)
+        t0 = time.perf_counter()
+        O.circumcenters(hb.tets, hb.particles)
+        O.volumes(ob)
+        O.iterate_cells(ob, a, mn, nm)
+        spent += time.perf_counter() - t0
+        cells += hb.num_orig
+        used += 1
+        if spent >= target_s:
+            break
+    total = sum(b.num_orig for b in blocks)
+    est = spent * total / cells
+    return {"est_full_s": est, "cores": 1, "kind": "port",
+            "sample": f"{used} of {len(blocks)} whole blocks ({cells} of {total} cells, "
+                      f"{spent:.1f}s of CPU work: circumcenters, volume() and IterateCells "
+                      f"per block, own deposits only), scaled by cell count; single thread"}
 
 
 def cpu_baseline(hb, grid, target_s):
@@ -83,7 +153,7 @@ def cpu_baseline(hb, grid, target_s):
         m = m2
     del ob_full
     G = int(np.prod(grid))
-    return {"value": G / est, "unit": "grid pts/s", "cores": 1, "kind": "port",
+    return {"cores": 1, "kind": "port",
             "sample": f"first {m} of {n} cells ({spent:.1f}s of CPU work: circumcenters of "
                       f"the same fraction of tets, volume() and IterateCells over those "
                       f"cells), scaled to all cells; single thread",
@@ -108,9 +178,10 @@ def exact_paths(run, grid, G, args):
             st = run.sync()
         ms = (time.perf_counter() - t0) * 1e3 / args.steps
         kms, _ = run.kernel_time()
-        outs[name] = (run.owner(0, G), run.fetch(0, ngrid=G)["density"])
+        outs[name] = (run.owner(0), run.fetch(0)["density"])
         res[name] = {"ms_per_step": ms, "grid_pts_per_s": G / (ms * 1e-3),
                      "owner_kernel_ms": kms, "fp64_fallbacks": int(st["fp64_fallbacks"]),
+                     "fp32_filter_failure_rate": float(st["fp64_fallbacks"]) / G,
                      "cells_deposited": int(st["cells_scan"] + st["cells_cic"])}
         log(f"[exact {name}] {ms:.3f} ms/step, owner kernel {kms:.3f} ms, "
             f"fallbacks {st['fp64_fallbacks']}")
@@ -135,6 +206,8 @@ def main():
     ap.add_argument("--no-exact", action="store_true",
                     help="skip the exact-Voronoi mixed / FP64 measurement")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-graph", action="store_true",
+                    help="time eager launches instead of graph replays (for tracing tools)")
     ap.add_argument("--save-block", default=None,
                     help="tessellate, write the block to this .npz and exit")
     ap.add_argument("--load-block", default=None,
@@ -146,26 +219,58 @@ def main():
         sys.exit(2)
 
     from tess2_amd import host
-    from tess2_amd.ingest import HostBlock  # noqa: F401
 
-    c, hb = setup(args.config, args.load_block)
+    c, blocks = setup(args.config, args.load_block)
     if args.save_block:
-        np.savez(args.save_block, particles=hb.particles, tets=hb.tets,
-                 vert_to_tet=hb.vert_to_tet, num_orig=hb.num_orig,
-                 bounds_min=hb.bounds_min, bounds_max=hb.bounds_max)
+        _save_blocks(args.save_block, blocks)
         log(f"[setup] wrote {args.save_block}")
         return
+    multi = len(blocks) > 1
     gsz = c["grid"]
     grid = (gsz, gsz, gsz)
-    G = gsz ** 3
     t0 = time.time()
-    run = host.StagedRun([hb])
+    run = host.StagedRun(blocks)
     log(f"[stage] H2D upload {time.time() - t0:.2f}s")
+    run.run(grid)
+    run.sync()
+    G = sum(run.ngrid)                 # grid points over all block grids
+    # eager pass: HIP events around the dominant kernel on the library stream
     for _ in range(args.warmup):
         run.run(grid)
         run.sync()
-    run.kernel_time()               # reset the per-kernel event accumulator
-    log(f"[run] warmup {args.warmup} done; timing {args.steps} steps")
+    run.kernel_time()
+    dev_eager = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        run.run(grid)
+        dev_eager.append(run.sync()["device_ms"])
+    ms_eager = (time.perf_counter() - t0) * 1e3 / args.steps
+    kern_ms, nk = run.kernel_time()
+    log(f"[eager] {ms_eager:.3f} ms/step wall, {np.mean(dev_eager):.3f} ms device, "
+        f"dominant kernel {kern_ms:.3f} ms avg over {nk}")
+    # steady state: the whole step replayed from a HIP graph
+    graph = 0 if args.no_graph else 1
+    run.set_option("graph", graph)
+    tune = None
+    if multi:
+        best, opts = run.autotune(reps=3)
+        tune = {"best_ms": best, **opts}
+        log(f"[autotune] {best:.3f} ms with {opts}")
+        run.set_option("graph", 0)          # kernel time of the chosen split
+        for _ in range(2):
+            run.run(grid)
+            run.sync()
+        run.kernel_time()
+        for _ in range(args.steps):
+            run.run(grid)
+            run.sync()
+        kern_ms, nk = run.kernel_time()
+        log(f"[autotune] dominant kernel {kern_ms:.3f} ms with the chosen split")
+        run.set_option("graph", graph)
+    for _ in range(max(1, args.warmup)):
+        run.run(grid)
+        run.sync()
+    log(f"[graph] warmup done; timing {args.steps} steps")
     dev = []
     t0 = time.perf_counter()
     for _ in range(args.steps):
@@ -173,36 +278,50 @@ def main():
         st = run.sync()
         dev.append(st["device_ms"])
     t1 = time.perf_counter()
-    kern_ms, nk = run.kernel_time()
     ms = (t1 - t0) * 1e3 / args.steps
-    log(f"[run] {ms:.3f} ms/step wall, {np.mean(dev):.3f} ms device, cell kernel "
-        f"{kern_ms:.3f} ms avg over {nk}")
-    N_all, N, T = hb.num_particles, hb.num_orig, hb.num_tets
+    log(f"[graph] {ms:.3f} ms/step wall, {np.mean(dev):.3f} ms device")
+    N_all = sum(b.num_particles for b in blocks)
+    N = sum(b.num_orig for b in blocks)
+    T = sum(b.num_tets for b in blocks)
     b_alg = 16 * N_all + 32 * T + 4 * N + 4 * G
     achieved = b_alg / (kern_ms * 1e-3) / 1e9
     traffic = None
     tf = os.path.join(ROOT, "profiles", f"traffic_{c['name']}.json")
     if os.path.exists(tf):
         traffic = json.load(open(tf)).get("hbm_bytes_per_launch")
+    kernel = "k_cells_ref_batched" if multi else "k_cells_ref"
+    cfg = {"workload": c["name"], "blocks": len(blocks), "particles": N, "ghosts": N_all - N,
+           "tets": T, "grid": list(grid), "grid_points": G, "assignment": "reference_scan",
+           "estimator": "DENSE_TESS", "steady_state": "hip_graph" if graph else "eager",
+           "device_ms_per_step": float(np.mean(dev)), "eager_ms_per_step": ms_eager,
+           "cells_deposited": int(st["cells_scan"] + st["cells_cic"]),
+           "cells_cic": int(st["cells_cic"]), "cells_deferred": int(st["cells_deferred"]),
+           "points_routed": int(st["points_routed"]),
+           "points_shared": int(st["points_shared"])}
+    if multi:
+        cfg["block_particles"] = [int(b.num_orig) for b in blocks]
+        cfg["autotune"] = tune
     out = {
         "metric": METRIC, "value": G / (ms * 1e-3), "unit": "grid pts/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
-        "data": "synthetic",
-        "config": {"workload": c["name"], "particles": N, "ghosts": N_all - N, "tets": T,
-                   "grid": list(grid), "assignment": "reference_scan",
-                   "estimator": "DENSE_TESS", "device_ms_per_step": float(np.mean(dev)),
-                   "cells_deposited": int(st["cells_scan"] + st["cells_cic"]),
-                   "cells_cic": int(st["cells_cic"])},
+        "data": "synthetic", "config": cfg,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "k_cells_ref", "kernel_ms": kern_ms, "alg_bytes": b_alg},
+                     "kernel": kernel, "kernel_ms": kern_ms, "alg_bytes": b_alg},
     }
-    if not args.no_exact:
+    run.set_option("graph", 0)
+    if not args.no_exact and not multi:
         out["exact_voronoi"] = exact_paths(run, grid, G, args)
     if not args.no_cpu:
         log("[cpu] timing the oracle on a sample")
-        out["cpu_baseline"] = cpu_baseline(hb, grid, args.cpu_seconds)
+        if multi:
+            cb = cpu_baseline_blocks(blocks, grid, args.cpu_seconds)
+        else:
+            cb = cpu_baseline(blocks[0], grid, args.cpu_seconds)
+        cb["value"] = G / cb["est_full_s"]
+        cb["unit"] = "grid pts/s"
+        out["cpu_baseline"] = cb
     print(json.dumps(out), flush=True)
 
 

@@ -94,13 +94,15 @@ typedef struct tess2_stats_t {
     long long cells_outside;
     long long cells_degenerate;
     long long points_deposited;      /* in-block deposits                     */
-    long long points_offblock;       /* points outside the block's bounds     */
-    long long points_offgrid;        /* in bounds but outside the block grid  */
+    long long points_offblock;       /* points no block grid holds (dropped)  */
+    long long points_routed;         /* points added to another block's grid  */
     long long points_shared;         /* deposits folded by the ordered pass   */
     long long fp64_fallbacks;        /* EXACT_VORONOI mixed: FP64 re-checks   */
     double tot_mass;                 /* sum of deposited point masses         */
     float max_dense;
     float device_ms;                 /* device time of the last run          */
+    long long cells_deferred;        /* cells above the one-wave kernel's star or face
+                                        capacity, handled by the large-capacity pass */
 } tess2_stats_t;
 
 typedef struct tess2_ctx tess2_ctx;
@@ -184,6 +186,29 @@ int tess2_fetch_owner(tess2_ctx *ctx, int b, int *owner);
  * runs since the last call, measured with HIP events on the library
  * stream, and the number of launches averaged. */
 int tess2_kernel_time(tess2_ctx *ctx, float *avg_ms, int *launches);
+
+/* ---- launch options and tuning (no counterpart in the reference: its
+ * dense() runs one OpenMP-free loop per block, src/dense.cpp:216-280) ---- */
+
+enum {
+    TESS2_OPT_GRAPH = 0,          /* 1: record the run's launches once into a HIP graph and
+                                   * replay it while arguments and buffers are unchanged */
+    TESS2_OPT_WAVES_PER_CU = 1,   /* persistent waves per CU of the multi-block cell kernel */
+    TESS2_OPT_CHUNK_SMALL = 2,    /* cells per work item, blocks with < 20000 cells */
+    TESS2_OPT_CHUNK_MEDIUM = 3,   /* cells per work item, blocks with < 100000 cells */
+    TESS2_OPT_CHUNK_LARGE = 4     /* cells per work item, larger blocks */
+};
+
+/* Set / read one option.  Chunk sizes 1..4096, waves per CU 1..64. */
+int tess2_set_option(tess2_ctx *ctx, int option, int value);
+int tess2_get_option(tess2_ctx *ctx, int option, int *value);
+
+/* Time the last tess2_run's arguments under each candidate work split of the
+ * multi-block cell kernel (chunk sizes x persistent waves per CU), `reps`
+ * runs each, keep the fastest, and leave it set.  Results of the staged
+ * blocks are those of the last timed run (identical for every candidate).
+ * best_ms (may be NULL) receives the fastest median run time. */
+int tess2_autotune(tess2_ctx *ctx, int reps, float *best_ms);
 
 #ifdef __cplusplus
 }

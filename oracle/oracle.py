@@ -116,8 +116,10 @@ class Block:
     @classmethod
     def from_host(cls, hb):
         """From a tess2_amd.ingest.HostBlock."""
-        return cls(hb.particles, hb.tets, hb.vert_to_tet, hb.num_orig,
-                   hb.bounds_min, hb.bounds_max)
+        b = cls(hb.particles, hb.tets, hb.vert_to_tet, hb.num_orig,
+                hb.bounds_min, hb.bounds_max)
+        b.gid = int(getattr(hb, "gid", 0))
+        return b
 
 
 def make_args(mass, data_mins, data_maxs, gmin, step, eps, gnum, project=False,
@@ -326,3 +328,50 @@ def exact_dense(block, gnum, mass=1.0, eps=1e-4):
     lib().o_exact_dense(C.byref(block.cs), C.byref(args), bmn, bnum, owner, status, npts, den)
     return dict(owner=owner, status=status, npts=npts, density=den, grid_step_size=step,
                 grid_phys_mins=gmin, block_num_idx=bnum, args=args)
+
+
+def dense_multi(blocks, gnum, mass=1.0, eps=1e-4, cic=False):
+    """dense() over several blocks (src/dense.cpp:30-103): each block deposits
+    its own cells' points that fall in its bounds and grid (IterateCells, in
+    cell order); the others are sent to the block whose grid (BlockGridParams
+    range) holds their global index and added there after its own deposits,
+    in ascending source gid, then source cell, order (recvd_pts,
+    src/dense.cpp:166-202).  Points no block holds are dropped.
+    blocks: list of Block with a .gid attribute.  Returns a dict."""
+    order = sorted(range(len(blocks)), key=lambda i: blocks[i].gid)
+    dmin = np.min([np.array(b.cs.bounds_min[:], np.float32) for b in blocks], axis=0)
+    dmax = np.max([np.array(b.cs.bounds_max[:], np.float32) for b in blocks], axis=0)
+    gmin, gmax, step = grid_step_params(dmin, dmax, gnum)
+    args = make_args(mass, dmin, dmax, gmin, step, eps, gnum)
+    grids = [block_grid_params(np.array(b.cs.bounds_min[:], np.float32),
+                               np.array(b.cs.bounds_max[:], np.float32), gmin, step, eps,
+                               dmin, dmax, gnum) for b in blocks]
+    dens, stat, npts, offs = [], [], [], []
+    for b, (mn, mx, nm) in zip(blocks, grids):
+        d, s, n, off, _ = iterate_cells(b, args, mn, nm, cic=cic)
+        dens.append(d)
+        stat.append(s)
+        npts.append(n)
+        offs.append(off)
+    recv = [[] for _ in blocks]
+    dropped = 0
+    for i in order:                       # source blocks by gid
+        idx3, m, cell = offs[i]
+        for k in range(idx3.shape[0]):
+            g = idx3[k]
+            dest = -1
+            for j, (mn, mx, nm) in enumerate(grids):
+                if np.all(g >= mn) and np.all(g <= mn + nm - 1):
+                    dest = j
+                    break
+            if dest < 0 or dest == i:
+                dropped += 1
+                continue
+            recv[dest].append((g, m[k]))
+    for j, (mn, mx, nm) in enumerate(grids):
+        if recv[j]:
+            idx = np.array([r[0] for r in recv[j]], np.int32)
+            mm = np.array([r[1] for r in recv[j]], np.float64)
+            recvd_pts(idx, mm, mn, nm, args, dens[j])
+    return dict(density=dens, status=stat, npts=npts, grids=grids, dropped=dropped,
+                grid_phys_mins=gmin, grid_step_size=step, args=args)

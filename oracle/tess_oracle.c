@@ -776,14 +776,20 @@ int o_cell_points(const o_block_t *b, const o_args_t *a, int cell, int *idx3,
 }
 
 /* deposit one point (src/dense.cpp:276-305): in-block -> density,
- * else -> off-block list.  Returns 1 if off-block. */
+ * else -> off-block list.  Returns 1 if off-block.  A point inside the
+ * block bounds whose local index falls outside the block grid (possible on a
+ * shared face, where BlockGridParams gives the index to the upper block; the
+ * reference would index out of its row there) is treated as off-block. */
 static int deposit(const o_block_t *b, const o_args_t *a, const int *bmin_idx,
                    const int *bnum_idx, float *density, const int *gidx,
                    double m, float div, double *tot_mass)
 {
     float pos[3];
     o_idx2phys(gidx, pos, a->grid_step_size, a->grid_phys_mins);
-    if (pos[0] >= b->bounds_min[0] && pos[0] <= b->bounds_max[0] &&
+    int lo[3] = {gidx[0] - bmin_idx[0], gidx[1] - bmin_idx[1], gidx[2] - bmin_idx[2]};
+    int ongrid = lo[0] >= 0 && lo[1] >= 0 && lo[0] < bnum_idx[0] && lo[1] < bnum_idx[1] &&
+                 (a->project || (lo[2] >= 0 && lo[2] < bnum_idx[2]));
+    if (ongrid && pos[0] >= b->bounds_min[0] && pos[0] <= b->bounds_max[0] &&
         pos[1] >= b->bounds_min[1] && pos[1] <= b->bounds_max[1] &&
         pos[2] >= b->bounds_min[2] && pos[2] <= b->bounds_max[2]) {
         int l[3] = {gidx[0] - bmin_idx[0], gidx[1] - bmin_idx[1],

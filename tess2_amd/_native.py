@@ -39,10 +39,10 @@ class Stats(C.Structure):
     _fields_ = [("cells_scan", C.c_longlong), ("cells_cic", C.c_longlong),
                 ("cells_incomplete", C.c_longlong), ("cells_outside", C.c_longlong),
                 ("cells_degenerate", C.c_longlong), ("points_deposited", C.c_longlong),
-                ("points_offblock", C.c_longlong), ("points_offgrid", C.c_longlong),
+                ("points_offblock", C.c_longlong), ("points_routed", C.c_longlong),
                 ("points_shared", C.c_longlong), ("fp64_fallbacks", C.c_longlong),
                 ("tot_mass", C.c_double), ("max_dense", C.c_float),
-                ("device_ms", C.c_float)]
+                ("device_ms", C.c_float), ("cells_deferred", C.c_longlong)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -78,7 +78,12 @@ SIGNATURES = {
     "tess2_fetch": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]),
     "tess2_kernel_time": (C.c_int, [C.c_void_p, F32P, I32P]),
     "tess2_fetch_owner": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p]),
+    "tess2_set_option": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
+    "tess2_get_option": (C.c_int, [C.c_void_p, C.c_int, I32P]),
+    "tess2_autotune": (C.c_int, [C.c_void_p, C.c_int, F32P]),
 }
+
+OPT_GRAPH, OPT_WAVES_PER_CU, OPT_CHUNK_SMALL, OPT_CHUNK_MEDIUM, OPT_CHUNK_LARGE = 0, 1, 2, 3, 4
 
 
 def lib():

@@ -40,6 +40,9 @@ struct DevBlock {
     int *head = nullptr;
     int4 *ovf = nullptr;
     unsigned ovf_cap = 0;
+    uint2 *seg = nullptr;                 // fold segments, 2 * ovf_cap
+    int4 *fold[3] = {nullptr, nullptr, nullptr};
+    unsigned fold_cap[3] = {0, 0, 0};
     int *slow = nullptr;
     unsigned slow_cap = 0;
     // exact-Voronoi assignment
@@ -50,15 +53,9 @@ struct DevBlock {
     int *hbk = nullptr;
     float4 *hsorted = nullptr;
     size_t owner_cap = 0;
-    unsigned *ctrs = nullptr;                // C_NCTR
-    unsigned long long *cstat = nullptr;     // 6 cell stats
-    double *gsum = nullptr;
-    unsigned *gmax = nullptr;
-    // host mirrors
-    unsigned h_ctrs[C_NCTR] = {};
-    unsigned long long h_cstat[6] = {};
-    double h_gsum = 0.0;
-    unsigned h_gmax = 0;
+    BlockStats *st = nullptr;                // this block's record in the context's array
+    unsigned *ctrs = nullptr;                // st->ctrs
+    BlockStats *hm = nullptr;                // its pinned host copy, refreshed by every run
     int block_max_idx[3] = {0, 0, 0};
 };
 
@@ -84,7 +81,65 @@ struct tess2_ctx {
     double ktime_sum = 0.0;
     int ktime_n = 0;
     float last_run_ms = 0.0f;
+    // multi-block: descriptors of all staged blocks on the device
+    int ncu = 256;
+    BlockIn *d_ins = nullptr;
+    GridP *d_gps = nullptr;
+    CellOut *d_outs = nullptr;
+    Router *d_router = nullptr;
+    Chunk *d_chunks = nullptr;
+    int nchunks = 0;
+    unsigned *d_queue = nullptr;
+    std::vector<BlockIn> h_ins;
+    std::vector<GridP> h_gps;
+    std::vector<CellOut> h_outs;
+    // batched schedule (tess2_autotune): chunk size per block size class and
+    // persistent waves per CU
+    int chunk_small = 16, chunk_medium = 32, chunk_large = 64, waves_per_cu = 28;
+    // graph replay of the steady-state sequence (tess2_set_option)
+    bool graph_on = false;
+    hipGraphExec_t gexec = nullptr;
+    RunArgs gargs;
+    unsigned long long gen = 1;      // bumped whenever a run buffer is reallocated
+    unsigned long long ggen = 0;     // generation the graph was captured at
+    bool in_capture = false;
+    BlockStats *d_stats = nullptr, *h_stats = nullptr;   // per-block counters, device / pinned
+    double *d_gpart = nullptr;                           // partial grid sums, 256 per block
+    bool kevents = false;            // the last run recorded the kernel events
+    RunArgs desc_args;
+    unsigned long long desc_gen = 0;
 };
+
+static bool same_args(const RunArgs &x, const RunArgs &y)
+{
+    return memcmp(&x, &y, sizeof(RunArgs)) == 0;
+}
+
+static void drop_graph(tess2_ctx *c)
+{
+    if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+    c->gexec = nullptr;
+    c->ggen = 0;
+}
+
+static void free_multi(tess2_ctx *c)
+{
+    void *ps[] = {c->d_ins, c->d_gps, c->d_outs, c->d_router, c->d_chunks, c->d_queue,
+                  c->d_stats, c->d_gpart};
+    for (void *p : ps)
+        if (p) (void)hipFree(p);
+    if (c->h_stats) (void)hipHostFree(c->h_stats);
+    c->d_stats = nullptr;
+    c->h_stats = nullptr;
+    c->d_gpart = nullptr;
+    c->d_ins = nullptr;
+    c->d_gps = nullptr;
+    c->d_outs = nullptr;
+    c->d_router = nullptr;
+    c->d_chunks = nullptr;
+    c->d_queue = nullptr;
+    c->nchunks = 0;
+}
 
 static int fail(tess2_ctx *c, int code, const char *fmt, ...)
 {
@@ -112,6 +167,7 @@ static int dalloc(tess2_ctx *c, T **p, size_t n)
         *p = nullptr;
     }
     if (n == 0) n = 1;
+    if (c) c->gen++;   // any captured graph refers to the old buffers
     hipError_t e = hipMalloc((void **)p, n * sizeof(T));
     if (e != hipSuccess) {
         *p = nullptr;
@@ -124,9 +180,9 @@ static int dalloc(tess2_ctx *c, T **p, size_t n)
 static void free_block(DevBlock &b)
 {
     void *ps[] = {b.order,  b.pts,    b.tets,   b.v2t,   b.cc,   b.vol,  b.status,    b.npts,
-                  b.density, b.count, b.first, b.head, b.ovf,  b.slow,      b.ctrs,
-                  b.cstat,  b.gsum,   b.gmax,  b.owner, b.status_ex, b.hstart, b.hfill,
-                  b.hsums,  b.hbk,    b.hsorted};
+                  b.density, b.count, b.first, b.head, b.ovf,  b.slow,
+                  b.owner, b.status_ex, b.hstart, b.hfill,
+                  b.hsums,  b.hbk,    b.hsorted, b.seg, b.fold[0], b.fold[1], b.fold[2]};
     for (void *p : ps)
         if (p) (void)hipFree(p);
     b = DevBlock();
@@ -200,6 +256,10 @@ extern "C" int tess2_create(tess2_ctx **out, int device)
         delete c;
         return TESS2_E_HIP;
     }
+    int ncu = 0;
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess &&
+        ncu > 0)
+        c->ncu = ncu;
     *out = c;
     return 0;
 }
@@ -209,7 +269,9 @@ extern "C" int tess2_destroy(tess2_ctx *c)
     if (!c) return 0;
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    drop_graph(c);
     for (auto &b : c->blocks) free_block(b);
+    free_multi(c);
     if (c->ev_run0) (void)hipEventDestroy(c->ev_run0);
     if (c->ev_run1) (void)hipEventDestroy(c->ev_run1);
     if (c->ev_k0) (void)hipEventDestroy(c->ev_k0);
@@ -319,6 +381,38 @@ static int validate(tess2_ctx *c, const tess2_block_t &b)
     return 0;
 }
 
+// Work list of the batched launch: each block's cells (in its Morton order)
+// cut into chunks whose size depends on the block's size class; blocks in
+// decreasing size so that the big ones start first.
+static int build_chunks(tess2_ctx *c)
+{
+    std::vector<int> ord(c->blocks.size());
+    for (size_t i = 0; i < ord.size(); ++i) ord[i] = (int)i;
+    std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) {
+        return c->blocks[a].norig > c->blocks[b].norig;
+    });
+    std::vector<Chunk> ch;
+    for (int b : ord) {
+        const int n = c->blocks[b].norig;
+        const int cs = n < 20000 ? c->chunk_small : (n < 100000 ? c->chunk_medium : c->chunk_large);
+        for (int s0 = 0; s0 < n; s0 += cs) {
+            Chunk k;
+            k.block = b;
+            k.start = s0;
+            k.count = std::min(cs, n - s0);
+            k.pad = 0;
+            ch.push_back(k);
+        }
+    }
+    c->nchunks = (int)ch.size();
+    int r = dalloc(c, &c->d_chunks, std::max<size_t>(ch.size(), 1));
+    if (r) return r;
+    if (!ch.empty())
+        HIPCHK(c, hipMemcpy(c->d_chunks, ch.data(), sizeof(Chunk) * ch.size(),
+                            hipMemcpyHostToDevice));
+    return 0;
+}
+
 extern "C" int tess2_stage(tess2_ctx *c, const tess2_block_t *blocks, int nblocks)
 {
     if (!c || (nblocks > 0 && !blocks) || nblocks < 0) return fail(c, TESS2_E_ARG, "bad args");
@@ -328,7 +422,20 @@ extern "C" int tess2_stage(tess2_ctx *c, const tess2_block_t *blocks, int nblock
         int r = validate(c, blocks[i]);
         if (r) return r;
     }
+    if (nblocks > 1) {
+        // deposit keys hold (source rank, cell) in 7 + 24 bits
+        if (nblocks > 127) return fail(c, TESS2_E_ARG, "at most 127 blocks per context");
+        for (int i = 0; i < nblocks; ++i) {
+            if (blocks[i].num_orig_particles >= (1 << 24))
+                return fail(c, TESS2_E_ARG, "multi-block runs need < 2^24 cells per block");
+            if (i > 0 && blocks[i].gid <= blocks[i - 1].gid)
+                return fail(c, TESS2_E_ARG, "blocks must be given in ascending gid order");
+        }
+    }
+    drop_graph(c);
+    c->gen++;
     for (auto &b : c->blocks) free_block(b);
+    free_multi(c);
     c->blocks.assign(nblocks, DevBlock());
     c->have_run = false;
     std::vector<float4> tmp;
@@ -347,8 +454,7 @@ extern "C" int tess2_stage(tess2_ctx *c, const tess2_block_t *blocks, int nblock
         if ((r = dalloc(c, &d.pts, (size_t)d.np)) || (r = dalloc(c, &d.tets, 2 * (size_t)d.nt)) ||
             (r = dalloc(c, &d.v2t, (size_t)d.np)) || (r = dalloc(c, &d.cc, (size_t)d.nt)) ||
             (r = dalloc(c, &d.vol, (size_t)d.norig)) || (r = dalloc(c, &d.status, (size_t)d.norig)) ||
-            (r = dalloc(c, &d.npts, (size_t)d.norig)) || (r = dalloc(c, &d.ctrs, (size_t)C_NCTR)) ||
-            (r = dalloc(c, &d.cstat, 6)) || (r = dalloc(c, &d.gsum, 1)) || (r = dalloc(c, &d.gmax, 1)))
+            (r = dalloc(c, &d.npts, (size_t)d.norig)))
             return r;
         tmp.resize(d.np);
         for (int p = 0; p < d.np; ++p)
@@ -408,6 +514,45 @@ extern "C" int tess2_stage(tess2_ctx *c, const tess2_block_t *blocks, int nblock
             (r = dalloc(c, &d.status_ex, (size_t)d.norig)))
             return r;
     }
+    if (nblocks > 0) {
+        int r = 0;
+        if ((r = dalloc(c, &c->d_ins, (size_t)nblocks)) || (r = dalloc(c, &c->d_gps, (size_t)nblocks)) ||
+            (r = dalloc(c, &c->d_outs, (size_t)nblocks)) || (r = dalloc(c, &c->d_router, 1)) ||
+            (r = dalloc(c, &c->d_queue, 4)) || (r = dalloc(c, &c->d_stats, (size_t)nblocks)) ||
+            (r = dalloc(c, &c->d_gpart, (size_t)nblocks * 256)))
+            return r;
+        // counters and statistics of all blocks: one device array, one pinned
+        // host copy (so that the copy can sit in a captured graph)
+        if (hipHostMalloc((void **)&c->h_stats, sizeof(BlockStats) * nblocks,
+                          hipHostMallocDefault) != hipSuccess)
+            return fail(c, TESS2_E_NOMEM, "pinned host buffer");
+        memset(c->h_stats, 0, sizeof(BlockStats) * nblocks);
+        for (int k = 0; k < nblocks; ++k) {
+            DevBlock &d = c->blocks[k];
+            d.st = c->d_stats + k;
+            d.ctrs = d.st->ctrs;
+            d.hm = c->h_stats + k;
+        }
+        r = build_chunks(c);
+        if (r) return r;
+    }
+    return 0;
+}
+
+// shared-point list and the fold's segment storage and class lists for
+// `cap` list entries (at most 2 * cap contributions in shared points)
+static int alloc_lists(tess2_ctx *c, DevBlock &d, unsigned cap)
+{
+    int r = 0;
+    const unsigned c1 = 2 * cap / (FOLD_SMALL_MAX + 1) + 1, c2 = 2 * cap / (FOLD_MID_MAX + 1) + 1;
+    if ((r = dalloc(c, &d.ovf, cap)) || (r = dalloc(c, &d.seg, 2 * (size_t)cap)) ||
+        (r = dalloc(c, &d.fold[0], cap)) || (r = dalloc(c, &d.fold[1], c1)) ||
+        (r = dalloc(c, &d.fold[2], c2)))
+        return r;
+    d.ovf_cap = cap;
+    d.fold_cap[0] = cap;
+    d.fold_cap[1] = c1;
+    d.fold_cap[2] = c2;
     return 0;
 }
 
@@ -423,8 +568,8 @@ static int ensure_grid(tess2_ctx *c, DevBlock &d, size_t n)
         d.grid_cap = n;
     }
     if (!d.ovf) {
-        d.ovf_cap = (unsigned)std::max<size_t>(1u << 16, n / 8);
-        int r = dalloc(c, &d.ovf, d.ovf_cap);
+        int r = alloc_lists(c, d, (unsigned)std::min<size_t>(std::max<size_t>(1u << 16, n / 8),
+                                                             1u << 30));
         if (r) return r;
     }
     return 0;
@@ -442,6 +587,14 @@ static CellOut make_out(DevBlock &d, bool want_vol)
     o.head = d.head;
     o.ovf = d.ovf;
     o.ovf_cap = d.ovf_cap;
+    o.seg = d.seg;
+    for (int k = 0; k < 3; ++k) {
+        o.fold[k] = d.fold[k];
+        o.fold_cap[k] = d.fold_cap[k];
+    }
+    o.st = d.st;
+    o.ngrid = d.ngrid;
+    o.ncells = d.norig;
     o.ctrs = d.ctrs;
     o.slow = d.slow;
     o.slow_cap = d.slow_cap;
@@ -462,13 +615,18 @@ static BlockIn make_in(DevBlock &d)
 }
 
 // enqueue the whole path for every staged block
-static int enqueue_run(tess2_ctx *c, const RunArgs &a)
+// host-side set-up of a run: grid parameters, buffers, device descriptors
+static int prepare_run(tess2_ctx *c, const RunArgs &a)
 {
     if (a.gnum[0] < 2 || a.gnum[1] < 2 || a.gnum[2] < 2)
         return fail(c, TESS2_E_ARG, "grid needs at least 2 points per axis");
     if (c->blocks.empty()) return fail(c, TESS2_E_ARG, "no staged blocks");
+    const int nb = (int)c->blocks.size();
+    const bool exact = a.alg == TESS2_DENSE_TESS && a.assignment == TESS2_ASSIGN_EXACT_VORONOI;
+    if (exact && nb > 1)
+        return fail(c, TESS2_E_ARG, "the exact assignment runs on one block at a time");
     // DataBounds(): min/max over the block bounds (src/dense.cpp:1238-1267)
-    for (size_t k = 0; k < c->blocks.size(); ++k)
+    for (int k = 0; k < nb; ++k)
         for (int i = 0; i < 3; ++i) {
             const DevBlock &b = c->blocks[k];
             if (k == 0 || b.bmin[i] < c->data_mins[i]) c->data_mins[i] = b.bmin[i];
@@ -478,9 +636,13 @@ static int enqueue_run(tess2_ctx *c, const RunArgs &a)
                            c->gmin, c->gmax, c->step, a.gnum);
     const float div = a.project ? c->step[0] * c->step[1] : c->step[0] * c->step[1] * c->step[2];
     hipStream_t s = c->stream;
-    HIPCHK(c, hipEventRecord(c->ev_run0, s));
-    bool first_block = true;
-    for (auto &d : c->blocks) {
+
+    // per-block grids and buffers
+    c->h_ins.resize(nb);
+    c->h_gps.resize(nb);
+    c->h_outs.resize(nb);
+    for (int k = 0; k < nb; ++k) {
+        DevBlock &d = c->blocks[k];
         GridP &gp = d.gp;
         int mx[3];
         tess2_block_grid_params(d.bmin, d.bmax, gp.bmin_idx, mx, gp.bnum_idx, c->gmin, c->step,
@@ -504,58 +666,146 @@ static int enqueue_run(tess2_ctx *c, const RunArgs &a)
                          (a.project ? 1 : (size_t)gp.bnum_idx[2]);
         int r = ensure_grid(c, d, n);
         if (r) return r;
-
-        launch_fill_u32(d.ctrs, C_NCTR, 0u, s);
-        launch_fill_u32((unsigned *)d.density, n, 0u, s);
-        launch_fill_u32(d.count, n, 0u, s);
-        launch_fill_u32((unsigned *)d.cstat, 12, 0u, s);
-        launch_fill_u32((unsigned *)d.gsum, 2, 0u, s);
-        launch_fill_u32(d.gmax, 1, 0u, s);
-        BlockIn in = make_in(d);
-        CellOut o = make_out(d, a.want_vol != 0);
-        if (a.alg == TESS2_DENSE_TESS && a.assignment == TESS2_ASSIGN_EXACT_VORONOI) {
-            if (d.owner_cap < n) {
-                r = dalloc(c, &d.owner, n);
-                if (r) return r;
-                d.owner_cap = n;
-            }
-            launch_circumcenters(d.pts, d.tets, d.nt, d.cc, s);
-            if (a.want_vol) {
-                // volume() as the reference computes it
-                CellOut ov = make_out(d, true);
-                launch_cells_ref(in, gp, ov, 0, s);
-                launch_cells_ref_slow(in, gp, ov, 0, s);
-            }
-            launch_hash_build(d.pts, d.np, d.hash, d.hstart, d.hfill, d.hsums, d.hbk, d.hsorted, s);
-            launch_exact_status(in, gp, d.status_ex, a.precision, d.ctrs, d.slow, d.slow_cap, s);
-            if (first_block) HIPCHK(c, hipEventRecord(c->ev_k0, s));
-            launch_owner(gp, d.hash, d.hstart, d.hsorted, d.owner, a.precision, d.ctrs,
-                         d.status_ex, d.npts, d.norig, s);
-            if (first_block) HIPCHK(c, hipEventRecord(c->ev_k1, s));
-            launch_exact_tail(in, gp, d.owner, d.status_ex, o, a.precision, s);
-        } else if (a.alg == TESS2_DENSE_CIC) {
-            if (first_block) HIPCHK(c, hipEventRecord(c->ev_k0, s));
-            launch_cic(in, gp, o, s);
-            if (first_block) HIPCHK(c, hipEventRecord(c->ev_k1, s));
-        } else {
-            launch_circumcenters(d.pts, d.tets, d.nt, d.cc, s);
-            if (first_block) HIPCHK(c, hipEventRecord(c->ev_k0, s));
-            launch_cells_ref(in, gp, o, 1, s);
-            if (first_block) HIPCHK(c, hipEventRecord(c->ev_k1, s));
-            launch_cells_ref_slow(in, gp, o, 1, s);
+        if (exact && d.owner_cap < n) {
+            r = dalloc(c, &d.owner, n);
+            if (r) return r;
+            d.owner_cap = n;
         }
-        launch_finalize(o, gp, s);
-        launch_cell_stats(d.status, d.npts, d.norig, d.cstat, s);
-        launch_grid_stats(d.density, n, d.gsum, d.gmax, s);
-        HIPCHK(c, hipGetLastError());
-        HIPCHK(c, hipMemcpyAsync(d.h_ctrs, d.ctrs, sizeof(unsigned) * C_NCTR,
-                                 hipMemcpyDeviceToHost, s));
-        HIPCHK(c, hipMemcpyAsync(d.h_cstat, d.cstat, sizeof(d.h_cstat), hipMemcpyDeviceToHost, s));
-        HIPCHK(c, hipMemcpyAsync(&d.h_gsum, d.gsum, sizeof(double), hipMemcpyDeviceToHost, s));
-        HIPCHK(c, hipMemcpyAsync(&d.h_gmax, d.gmax, sizeof(unsigned), hipMemcpyDeviceToHost, s));
-        first_block = false;
+        c->h_ins[k] = make_in(d);
+        c->h_gps[k] = gp;
+        c->h_outs[k] = make_out(d, a.want_vol != 0);
     }
-    HIPCHK(c, hipEventRecord(c->ev_run1, s));
+    if (c->desc_gen == c->gen && same_args(a, c->desc_args)) return 0;   // unchanged
+    Router hr;
+    hr.nblocks = nb;
+    hr.gps = c->d_gps;
+    hr.outs = c->d_outs;
+    HIPCHK(c, hipMemcpyAsync(c->d_ins, c->h_ins.data(), sizeof(BlockIn) * nb,
+                             hipMemcpyHostToDevice, s));
+    HIPCHK(c, hipMemcpyAsync(c->d_gps, c->h_gps.data(), sizeof(GridP) * nb,
+                             hipMemcpyHostToDevice, s));
+    HIPCHK(c, hipMemcpyAsync(c->d_outs, c->h_outs.data(), sizeof(CellOut) * nb,
+                             hipMemcpyHostToDevice, s));
+    HIPCHK(c, hipMemcpyAsync(c->d_router, &hr, sizeof(Router), hipMemcpyHostToDevice, s));
+    // the host copies must outlive the asynchronous transfers
+    HIPCHK(c, hipStreamSynchronize(s));
+    c->desc_gen = c->gen;
+    c->desc_args = a;
+    return 0;
+}
+
+// the kernel sequence of a run (captured into a graph in graph mode)
+static int enqueue_kernels(tess2_ctx *c, const RunArgs &a)
+{
+    const int nb = (int)c->blocks.size();
+    const bool exact = a.alg == TESS2_DENSE_TESS && a.assignment == TESS2_ASSIGN_EXACT_VORONOI;
+    hipStream_t s = c->stream;
+    const bool ev = !c->in_capture;
+
+    // phase A: clear per-block outputs and counters, circumcenters
+    launch_fill_u32(reinterpret_cast<unsigned *>(c->d_stats), sizeof(BlockStats) / 4 * nb, 0u, s);
+    {
+        int mt = 0;
+        unsigned long long mg = 0;
+        for (const DevBlock &d : c->blocks) {
+            mt = std::max(mt, d.nt);
+            mg = std::max<unsigned long long>(mg, d.ngrid);
+        }
+        launch_block_prep(c->d_ins, c->d_outs, nb, mt, mg, a.alg == TESS2_DENSE_TESS, s);
+    }
+    // phase B: the estimator
+    if (exact) {
+        DevBlock &d = c->blocks[0];
+        const BlockIn &in = c->h_ins[0];
+        const GridP &gp = c->h_gps[0];
+        if (a.want_vol) {
+            // volume() as the reference computes it
+            CellOut ov = make_out(d, true);
+            launch_cells_ref(in, gp, ov, 0, s);
+            launch_cells_ref_slow(in, gp, ov, 0, s);
+        }
+        launch_hash_build(d.pts, d.np, d.hash, d.hstart, d.hfill, d.hsums, d.hbk, d.hsorted, s);
+        launch_exact_status(in, gp, d.status_ex, a.precision, d.ctrs, d.slow, d.slow_cap, s);
+        if (ev) HIPCHK(c, hipEventRecord(c->ev_k0, s));
+        launch_owner(gp, d.hash, d.hstart, d.hsorted, d.owner, a.precision, d.ctrs, d.status_ex,
+                     d.npts, d.norig, s);
+        if (ev) HIPCHK(c, hipEventRecord(c->ev_k1, s));
+        launch_exact_tail(in, gp, d.owner, d.status_ex, c->h_outs[0], a.precision, s);
+    } else if (a.alg == TESS2_DENSE_CIC) {
+        if (ev) HIPCHK(c, hipEventRecord(c->ev_k0, s));
+        for (int k = 0; k < nb; ++k)
+            launch_cic(c->h_ins[k], c->h_gps[k], c->h_outs[k], nb > 1 ? c->d_router : nullptr, k,
+                       s);
+        if (ev) HIPCHK(c, hipEventRecord(c->ev_k1, s));
+    } else if (nb == 1) {
+        if (ev) HIPCHK(c, hipEventRecord(c->ev_k0, s));
+        launch_cells_ref(c->h_ins[0], c->h_gps[0], c->h_outs[0], 1, s);
+        if (ev) HIPCHK(c, hipEventRecord(c->ev_k1, s));
+        launch_cells_ref_slow(c->h_ins[0], c->h_gps[0], c->h_outs[0], 1, s);
+    } else {
+        if (ev) HIPCHK(c, hipEventRecord(c->ev_k0, s));
+        launch_cells_ref_batched(c->d_ins, c->d_gps, c->d_outs, c->d_router, c->d_chunks,
+                                 c->nchunks, c->d_queue, c->ncu * c->waves_per_cu, 1, s);
+        if (ev) HIPCHK(c, hipEventRecord(c->ev_k1, s));
+    }
+    // phase C: ordered fold of shared points, statistics, one copy of the
+    // counters of all blocks
+    unsigned mcap = 0, mfold[3] = {0, 0, 0};
+    int mcells = 0;
+    unsigned long long mgrid = 0;
+    for (const DevBlock &d : c->blocks) {
+        mcap = std::max(mcap, d.ovf_cap);
+        for (int k = 0; k < 3; ++k) mfold[k] = std::max(mfold[k], d.fold_cap[k]);
+        mcells = std::max(mcells, d.norig);
+        mgrid = std::max<unsigned long long>(mgrid, d.ngrid);
+    }
+    launch_fold(c->d_outs, c->d_gps, nb, mcap, mfold, s);
+    launch_stats(c->d_outs, c->d_gpart, nb, mcells, mgrid, s);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipMemcpyAsync(c->h_stats, c->d_stats, sizeof(BlockStats) * nb,
+                             hipMemcpyDeviceToHost, s));
+    return 0;
+}
+
+static int enqueue_run(tess2_ctx *c, const RunArgs &a)
+{
+    int r = prepare_run(c, a);
+    if (r) return r;
+    hipStream_t s = c->stream;
+    if (c->graph_on) {
+        if (!c->gexec || c->ggen != c->gen || !same_args(a, c->gargs)) {
+            drop_graph(c);
+            hipGraph_t g = nullptr;
+            HIPCHK(c, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+            c->in_capture = true;
+            r = enqueue_kernels(c, a);
+            c->in_capture = false;
+            hipError_t e = hipStreamEndCapture(s, &g);
+            if (r) {
+                if (g) (void)hipGraphDestroy(g);
+                return r;
+            }
+            if (e != hipSuccess) return fail(c, TESS2_E_HIP, "graph capture: %s", hipGetErrorString(e));
+            e = hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            if (e != hipSuccess) {
+                c->gexec = nullptr;
+                return fail(c, TESS2_E_HIP, "graph instantiate: %s", hipGetErrorString(e));
+            }
+            c->ggen = c->gen;
+            c->gargs = a;
+        }
+        HIPCHK(c, hipEventRecord(c->ev_run0, s));
+        HIPCHK(c, hipGraphLaunch(c->gexec, s));
+        HIPCHK(c, hipEventRecord(c->ev_run1, s));
+        c->kevents = false;
+    } else {
+        HIPCHK(c, hipEventRecord(c->ev_run0, s));
+        r = enqueue_kernels(c, a);
+        if (r) return r;
+        HIPCHK(c, hipEventRecord(c->ev_run1, s));
+        c->kevents = true;
+    }
     c->last = a;
     c->have_run = true;
     return 0;
@@ -608,23 +858,30 @@ extern "C" int tess2_sync(tess2_ctx *c, tess2_stats_t *st)
         if (!c->have_run) return fail(c, TESS2_E_ARG, "no run to wait for");
         bool retry = false;
         for (auto &d : c->blocks) {
-            if (d.h_ctrs[C_ERR] & 4u)
+            if (d.hm->ctrs[C_ERR] & 4u)
                 return fail(c, TESS2_E_CAPACITY,
                             "block %d: a cell exceeds the per-cell capacities", d.gid);
-            if (d.h_ctrs[C_ERR] & 1u) {
-                const unsigned want = std::max(2 * d.slow_cap, d.h_ctrs[C_SLOW] + 1024u);
+            if (d.hm->ctrs[C_ERR] & 1u) {
+                const unsigned want = std::max(2 * d.slow_cap, d.hm->ctrs[C_SLOW] + 1024u);
                 int r = dalloc(c, &d.slow, want);
                 if (r) return r;
                 d.slow_cap = want;
                 retry = true;
             }
-            if (d.h_ctrs[C_OVF] > d.ovf_cap) {
-                const unsigned want = std::max(2 * d.ovf_cap, d.h_ctrs[C_OVF] + d.h_ctrs[C_OVF] / 4);
-                int r = dalloc(c, &d.ovf, want);
+            if (d.hm->ctrs[C_OVF] > d.ovf_cap) {
+                const unsigned long long want =
+                    std::max<unsigned long long>(2ull * d.ovf_cap,
+                                                 d.hm->ctrs[C_OVF] + d.hm->ctrs[C_OVF] / 4ull);
+                if (want > (1ull << 30))
+                    return fail(c, TESS2_E_CAPACITY, "block %d: %u shared grid contributions",
+                                d.gid, d.hm->ctrs[C_OVF]);
+                int r = alloc_lists(c, d, (unsigned)want);
                 if (r) return r;
-                d.ovf_cap = want;
                 retry = true;
             }
+            if (d.hm->ctrs[C_ERR] & 8u)
+                return fail(c, TESS2_E_CAPACITY, "block %d: shared grid point beyond the fold's "
+                            "capacities", d.gid);
         }
         if (!retry) break;
         if (attempt >= 4) return fail(c, TESS2_E_CAPACITY, "list capacities did not converge");
@@ -633,31 +890,135 @@ extern "C" int tess2_sync(tess2_ctx *c, tess2_stats_t *st)
     }
     float ms = 0.0f, kms = 0.0f;
     HIPCHK(c, hipEventElapsedTime(&ms, c->ev_run0, c->ev_run1));
-    HIPCHK(c, hipEventElapsedTime(&kms, c->ev_k0, c->ev_k1));
     c->last_run_ms = ms;
-    c->ktime_sum += kms;
-    c->ktime_n += 1;
+    if (c->kevents) {
+        HIPCHK(c, hipEventElapsedTime(&kms, c->ev_k0, c->ev_k1));
+        c->ktime_sum += kms;
+        c->ktime_n += 1;
+    }
     if (st) {
         memset(st, 0, sizeof *st);
         for (auto &d : c->blocks) {
-            st->cells_scan += (long long)d.h_cstat[0];
-            st->cells_cic += (long long)d.h_cstat[1];
-            st->cells_incomplete += (long long)d.h_cstat[2];
-            st->cells_outside += (long long)d.h_cstat[3];
-            st->cells_degenerate += (long long)d.h_cstat[4];
-            st->points_offblock += d.h_ctrs[C_OFFBLOCK];
-            st->points_offgrid += d.h_ctrs[C_OFFGRID];
-            st->points_deposited += (long long)d.h_cstat[5] - d.h_ctrs[C_OFFBLOCK] -
-                                    d.h_ctrs[C_OFFGRID];
-            st->points_shared += d.h_ctrs[C_OVF];
-            st->fp64_fallbacks += d.h_ctrs[C_FALLBACK];
-            st->tot_mass += d.h_gsum * (double)d.gp.div;
+            st->cells_scan += (long long)d.hm->cstat[0];
+            st->cells_cic += (long long)d.hm->cstat[1];
+            st->cells_incomplete += (long long)d.hm->cstat[2];
+            st->cells_outside += (long long)d.hm->cstat[3];
+            st->cells_degenerate += (long long)d.hm->cstat[4];
+            st->points_offblock += d.hm->ctrs[C_OFFBLOCK];
+            st->points_routed += d.hm->ctrs[C_ROUTED];
+            st->points_deposited += (long long)d.hm->cstat[5] - d.hm->ctrs[C_OFFBLOCK];
+            st->points_shared += d.hm->ctrs[C_OVF];
+            st->fp64_fallbacks += d.hm->ctrs[C_FALLBACK];
+            st->cells_deferred += d.hm->ctrs[C_SLOW];
+            st->tot_mass += d.hm->gsum * (double)d.gp.div;
             float mx;
-            memcpy(&mx, &d.h_gmax, sizeof mx);
+            memcpy(&mx, &d.hm->gmax, sizeof mx);
             st->max_dense = std::max(st->max_dense, mx);
         }
         st->device_ms = ms;
     }
+    return 0;
+}
+
+extern "C" int tess2_set_option(tess2_ctx *c, int option, int value)
+{
+    if (!c) return TESS2_E_ARG;
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    int *chunk = nullptr;
+    switch (option) {
+    case TESS2_OPT_GRAPH:
+        if (value != 0 && value != 1) return fail(c, TESS2_E_ARG, "graph option is 0 or 1");
+        if (!value) drop_graph(c);
+        c->graph_on = value != 0;
+        return 0;
+    case TESS2_OPT_WAVES_PER_CU:
+        if (value < 1 || value > 64) return fail(c, TESS2_E_ARG, "waves per CU: 1..64");
+        if (value != c->waves_per_cu) c->gen++;
+        c->waves_per_cu = value;
+        return 0;
+    case TESS2_OPT_CHUNK_SMALL: chunk = &c->chunk_small; break;
+    case TESS2_OPT_CHUNK_MEDIUM: chunk = &c->chunk_medium; break;
+    case TESS2_OPT_CHUNK_LARGE: chunk = &c->chunk_large; break;
+    default: return fail(c, TESS2_E_ARG, "unknown option %d", option);
+    }
+    if (value < 1 || value > 4096) return fail(c, TESS2_E_ARG, "chunk size: 1..4096");
+    if (*chunk == value) return 0;
+    *chunk = value;
+    return c->blocks.size() > 1 ? build_chunks(c) : 0;
+}
+
+extern "C" int tess2_get_option(tess2_ctx *c, int option, int *value)
+{
+    if (!c || !value) return TESS2_E_ARG;
+    switch (option) {
+    case TESS2_OPT_GRAPH: *value = c->graph_on ? 1 : 0; return 0;
+    case TESS2_OPT_WAVES_PER_CU: *value = c->waves_per_cu; return 0;
+    case TESS2_OPT_CHUNK_SMALL: *value = c->chunk_small; return 0;
+    case TESS2_OPT_CHUNK_MEDIUM: *value = c->chunk_medium; return 0;
+    case TESS2_OPT_CHUNK_LARGE: *value = c->chunk_large; return 0;
+    default: return fail(c, TESS2_E_ARG, "unknown option %d", option);
+    }
+}
+
+// median device time of `reps` runs of the last arguments
+static int time_runs(tess2_ctx *c, int reps, float *med)
+{
+    std::vector<float> t;
+    for (int i = 0; i < reps; ++i) {
+        int r = enqueue_run(c, c->last);
+        if (r) return r;
+        r = tess2_sync(c, nullptr);
+        if (r) return r;
+        t.push_back(c->last_run_ms);
+    }
+    std::sort(t.begin(), t.end());
+    *med = t[t.size() / 2];
+    return 0;
+}
+
+extern "C" int tess2_autotune(tess2_ctx *c, int reps, float *best_ms)
+{
+    if (!c) return TESS2_E_ARG;
+    if (!c->have_run) return fail(c, TESS2_E_ARG, "autotune needs a previous tess2_run");
+    if (reps < 1 || reps > 100) return fail(c, TESS2_E_ARG, "reps: 1..100");
+    HIPCHK(c, hipSetDevice(c->device));
+    float best = 0.0f;
+    int r = time_runs(c, 1, &best);   // warm-up (and capture, in graph mode)
+    if (r) return r;
+    r = time_runs(c, reps, &best);
+    if (r) return r;
+    if (c->blocks.size() > 1) {
+        // chunk sizes for (small, medium, large) blocks and persistent waves
+        static const int chunks[][3] = {{8, 16, 32}, {16, 32, 64}, {32, 64, 128}, {64, 128, 256}};
+        static const int waves[] = {8, 16, 24, 32};
+        int bc[3] = {c->chunk_small, c->chunk_medium, c->chunk_large}, bw = c->waves_per_cu;
+        for (const auto &ch : chunks)
+            for (int w : waves) {
+                if ((r = tess2_set_option(c, TESS2_OPT_CHUNK_SMALL, ch[0])) ||
+                    (r = tess2_set_option(c, TESS2_OPT_CHUNK_MEDIUM, ch[1])) ||
+                    (r = tess2_set_option(c, TESS2_OPT_CHUNK_LARGE, ch[2])) ||
+                    (r = tess2_set_option(c, TESS2_OPT_WAVES_PER_CU, w)))
+                    return r;
+                float t = 0.0f;
+                if ((r = time_runs(c, 1, &t)) || (r = time_runs(c, reps, &t))) return r;
+                if (t < best) {
+                    best = t;
+                    bc[0] = ch[0];
+                    bc[1] = ch[1];
+                    bc[2] = ch[2];
+                    bw = w;
+                }
+            }
+        if ((r = tess2_set_option(c, TESS2_OPT_CHUNK_SMALL, bc[0])) ||
+            (r = tess2_set_option(c, TESS2_OPT_CHUNK_MEDIUM, bc[1])) ||
+            (r = tess2_set_option(c, TESS2_OPT_CHUNK_LARGE, bc[2])) ||
+            (r = tess2_set_option(c, TESS2_OPT_WAVES_PER_CU, bw)))
+            return r;
+        float t = 0.0f;
+        if ((r = time_runs(c, 1, &t))) return r;   // leave the results of the chosen split
+    }
+    if (best_ms) *best_ms = best;
     return 0;
 }
 
@@ -761,10 +1122,10 @@ extern "C" int tess2_volumes(tess2_ctx *c, const tess2_block_t *block, float *ou
     launch_cells_ref(in, gp, o, 0, s);
     launch_cells_ref_slow(in, gp, o, 0, s);
     HIPCHK(c, hipGetLastError());
-    HIPCHK(c, hipMemcpyAsync(d.h_ctrs, d.ctrs, sizeof(unsigned) * C_NCTR, hipMemcpyDeviceToHost,
+    HIPCHK(c, hipMemcpyAsync(d.hm->ctrs, d.ctrs, sizeof(unsigned) * C_NCTR, hipMemcpyDeviceToHost,
                              s));
     HIPCHK(c, hipStreamSynchronize(s));
-    if (d.h_ctrs[C_ERR] & 1u) {
+    if (d.hm->ctrs[C_ERR] & 1u) {
         // more deferred cells than the list holds: one cell per launch slot
         int rr = dalloc(c, &d.slow, (size_t)d.norig);
         if (rr) return rr;
@@ -773,11 +1134,11 @@ extern "C" int tess2_volumes(tess2_ctx *c, const tess2_block_t *block, float *ou
         launch_fill_u32(d.ctrs, C_NCTR, 0u, s);
         launch_cells_ref(in, gp, o, 0, s);
         launch_cells_ref_slow(in, gp, o, 0, s);
-        HIPCHK(c, hipMemcpyAsync(d.h_ctrs, d.ctrs, sizeof(unsigned) * C_NCTR,
+        HIPCHK(c, hipMemcpyAsync(d.hm->ctrs, d.ctrs, sizeof(unsigned) * C_NCTR,
                                  hipMemcpyDeviceToHost, s));
         HIPCHK(c, hipStreamSynchronize(s));
     }
-    if (d.h_ctrs[C_ERR] & 4u)
+    if (d.hm->ctrs[C_ERR] & 4u)
         return fail(c, TESS2_E_CAPACITY, "a cell exceeds the per-cell capacities");
     return tess2_fetch(c, 0, nullptr, out, nullptr);
 }

@@ -4,8 +4,9 @@
 //   1. star: level-synchronous BFS over the tets around the site, starting at
 //      vert_to_tet[v], that visits tets in exactly the first-seen order of
 //      neighbor_edges() (src/tet.cpp:228-270); each BFS level is one round of
-//      parallel gathers, duplicates are resolved with wave ballots.  Star tet
-//      records and their circumcenters are staged in LDS.
+//      parallel gathers, duplicates are resolved through a tet-id -> slot
+//      hash table in LDS.  Star tet records and their circumcenters are
+//      staged in LDS.
 //   2. faces: the Delaunay neighbours u in first-seen order with their first
 //      tet ut (one face per lane); each lane walks its edge ring in LDS
 //      (fill_edge_link, src/tet.cpp:389-409) and accumulates, in ring order,
@@ -49,15 +50,22 @@ __device__ __forceinline__ int comp(const int4 &q, int i)
 // the per-cell wavefront kernel
 //   NS   : star capacity in units of 64 tets
 //   NF   : face capacity in units of 64 faces
-//   RMAX : capacity of the scan's run list
+//   RMAX : scan runs kept in LDS (a cell with more scans twice, see step 5)
 // ---------------------------------------------------------------------------
+// LDS ints of the star / face tables: HS keys, HS states, 2 x SMAX slots
+template <int NS> constexpr int hash_entries() { return 2 * 64 * NS; }
+template <int NS> constexpr int hash_ints() { return 2 * hash_entries<NS>() + 2 * 64 * NS; }
+
 template <int NS, int NF, int RMAX>
 __device__ void cell_ref_body(const int v, const BlockIn &in, const GridP &gp,
                               const CellOut &o, const int want_density, const bool can_defer,
-                              int4 *s_v, int *s_ns, float4 *s_cc, int4 *s_u)
+                              int4 *s_v, int *s_ns, float4 *s_cc, int4 *s_u, int *s_h,
+                              const Router *rt = nullptr, const int self = 0)
 {
     constexpr int SMAX = 64 * NS;
     constexpr int FMAX = 64 * NF;
+    constexpr int HS = hash_entries<NS>();
+    static_assert(FMAX <= SMAX, "face lists reuse the slot arrays");
     constexpr int LMAX = 4096;
     const int lane = lane_id();
 
@@ -72,14 +80,23 @@ __device__ void cell_ref_body(const int v, const BlockIn &in, const GridP &gp,
     }
 
     // ---- 1. star BFS -------------------------------------------------------
-    int sid[NS];
-#pragma unroll
-    for (int k = 0; k < NS; ++k) sid[k] = -1;
+    // Tet ids map to star slots through an open-addressing table in LDS.  A
+    // batch of up to 64 candidates resolves at once: every lane finds or
+    // inserts its tet, an atomicMin leaves the lowest candidate key on each
+    // tet first seen in this batch, and those leaders take the next slots in
+    // key order -- the order in which neighbor_edges() first meets them.
+    int *hk = s_h, *hst = s_h + HS, *s_tid = s_h + 2 * HS, *s_aux = s_tid + SMAX;
+    for (int i = lane; i < HS; i += 64) {
+        hk[i] = -1;
+        hst[i] = 0x7fffffff;
+    }
+    __syncthreads();
     if (lane == 0) {
         s_v[0] = in.tets[2 * vt];
         s_u[0] = in.tets[2 * vt + 1];
         s_cc[0] = in.cc[vt];
-        sid[0] = vt;
+        hst[hash_find<HS>(hk, vt)] = -1;   // slot 0
+        s_tid[0] = vt;
     }
     int S = 1, f0 = 0, f1 = 1;
     bool hull = false, over = false;
@@ -107,47 +124,33 @@ __device__ void cell_ref_body(const int v, const BlockIn &in, const GridP &gp,
                 hull = true;
                 break;
             }
-            unsigned long long pend = __ballot(valid);
-            int myslot = -1;
-            while (pend) {
-                const int leader = __ffsll((unsigned long long)pend) - 1;
-                const int val = rdl(cand, leader);
-                const unsigned long long same = __ballot(valid && cand == val);
-                int slot = -1;
-#pragma unroll
-                for (int k = 0; k < NS; ++k) {
-                    const unsigned long long m = __ballot(sid[k] == val);
-                    if (slot < 0 && m) slot = k * 64 + __ffsll(m) - 1;
-                }
-                if (slot < 0) {
-                    slot = S;
-                    if (S < SMAX) {
-#pragma unroll
-                        for (int k = 0; k < NS; ++k)
-                            if (k == (S >> 6) && lane == (S & 63)) sid[k] = val;
-                    }
-                    S++;
-                }
-                if ((same >> lane) & 1ull) myslot = slot;
-                pend &= ~same;
+            const int e = valid ? hash_find<HS>(hk, cand) : 0;
+            if (__ballot(valid && e < 0)) {   // table full: the star is too large
+                over = true;
+                break;
             }
-            if (key < ncand) s_ns[s * 4 + i] = valid ? myslot : -1;
+            if (valid) atomicMin(&hst[e], key);
+            const bool lead = valid && hst[e] == key;
+            const unsigned long long L = __ballot(lead);
+            if (lead) {
+                const int slot = S + __popcll(L & ((1ull << lane) - 1ull));
+                hst[e] = -(slot + 1);
+                if (slot < SMAX) s_tid[slot] = cand;
+            }
+            S += __popcll(L);
+            if (key < ncand) s_ns[s * 4 + i] = valid ? -hst[e] - 1 : -1;
+            if (S > SMAX) {
+                over = true;
+                break;
+            }
         }
-        if (hull) break;
-        if (S > SMAX) {
-            over = true;
-            break;
-        }
+        if (hull || over) break;
+        __syncthreads();
         // gather the records of the new slots [f1, S)
         for (int b = f1; b < S; b += 64) {
             const int s = b + lane;
-            int id = -1;
-#pragma unroll
-            for (int k = 0; k < NS; ++k) {
-                const int t = __shfl(sid[k], s & 63);
-                if ((s >> 6) == k) id = t;
-            }
             if (s < S) {
+                const int id = s_tid[s];
                 s_v[s] = in.tets[2 * id];
                 s_u[s] = in.tets[2 * id + 1];
                 s_cc[s] = in.cc[id];
@@ -169,12 +172,14 @@ __device__ void cell_ref_body(const int v, const BlockIn &in, const GridP &gp,
 
     {
         // ---- 2. faces: neighbours in first-seen order -----------------------
-        int fu[NF], fs[NF];
-#pragma unroll
-        for (int j = 0; j < NF; ++j) {
-            fu[j] = -1;
-            fs[j] = 0;
+        // same table scheme keyed by vertex id; the entry key e = 4*slot + i
+        // orders first sightings, and a face's first tet is e >> 2
+        for (int i = lane; i < HS; i += 64) {
+            hk[i] = -1;
+            hst[i] = 0x7fffffff;
         }
+        int *s_fu = s_tid, *s_fs = s_aux;
+        __syncthreads();
         int F = 0;
         const int nent = S * 4;
         for (int base = 0; base < nent; base += 64) {
@@ -185,30 +190,34 @@ __device__ void cell_ref_body(const int v, const BlockIn &in, const GridP &gp,
                 u = comp(s_v[e >> 2], e & 3);
                 valid = (u != v);
             }
-            unsigned long long pend = __ballot(valid);
-            while (pend) {
-                const int leader = __ffsll((unsigned long long)pend) - 1;
-                const int val = rdl(u, leader);
-                const unsigned long long same = __ballot(valid && u == val);
-                bool found = false;
-#pragma unroll
-                for (int j = 0; j < NF; ++j)
-                    if (__ballot(fu[j] == val)) found = true;
-                if (!found) {
-                    if (F < FMAX) {
-#pragma unroll
-                        for (int j = 0; j < NF; ++j)
-                            if (j == (F >> 6) && lane == (F & 63)) {
-                                fu[j] = val;
-                                fs[j] = (base + leader) >> 2;
-                            }
-                    }
-                    F++;
-                }
-                pend &= ~same;
+            const int hx = valid ? hash_find<HS>(hk, u) : 0;
+            if (__ballot(valid && hx < 0)) {
+                F = FMAX + 1;
+                break;
             }
+            if (valid) atomicMin(&hst[hx], e);
+            const bool lead = valid && hst[hx] == e;
+            const unsigned long long L = __ballot(lead);
+            if (lead) {
+                const int f = F + __popcll(L & ((1ull << lane) - 1ull));
+                hst[hx] = -(f + 1);
+                if (f < FMAX) {
+                    s_fu[f] = u;
+                    s_fs[f] = e >> 2;
+                }
+            }
+            F += __popcll(L);
+            if (F > FMAX) break;
         }
         if (F > FMAX) goto defer;
+        __syncthreads();
+        int fu[NF], fs[NF];
+#pragma unroll
+        for (int j = 0; j < NF; ++j) {
+            const int f = j * 64 + lane;
+            fu[j] = f < F ? s_fu[f] : -1;
+            fs[j] = f < F ? s_fs[f] : 0;
+        }
 
         // ---- ring walks: Newell normal, fan area, bbox (one face per lane) --
         const float4 site = in.pts[v];
@@ -396,87 +405,185 @@ __device__ void cell_ref_body(const int v, const BlockIn &in, const GridP &gp,
             return !(__ballot(pos) != 0ull && __ballot(neg) != 0ull);
         };
 
+        // Packed PtInCell: with F <= 32 faces the wave tests G = 64 / gs points
+        // at once, lane group g (gs lanes, one face each) holding point g.  The
+        // per-face arithmetic is that of inside(), so the verdicts are the
+        // same; the scan reads them from a per-row cache (bit x = verdict at
+        // x) filled a window of G points at a time.  PtInCell is pure, so
+        // testing points the scan never asks for changes nothing.
+        int gs = 64;
+        if (NF == 1) gs = F <= 8 ? 8 : (F <= 16 ? 16 : (F <= 32 ? 32 : 64));
+        const int G = 64 / gs;
+        const int gshift = gs == 8 ? 3 : (gs == 16 ? 4 : (gs == 32 ? 5 : 6));
+        const int fl = lane & (gs - 1), grp = lane >> gshift;
+        const float rnx = __shfl(fnx[0], fl), rny = __shfl(fny[0], fl), rnz = __shfl(fnz[0], fl);
+        const float rfx = __shfl(ffx[0], fl), rfy = __shfl(ffy[0], fl), rfz = __shfl(ffz[0], fl);
+        const unsigned long long gmask = gs == 64 ? ~0ull : ((1ull << gs) - 1ull);
+        // verdicts (bit g) of the points (xs + dx*g, ys + dy*g, zs), g < G
+        auto packed = [&](int xs, int ys, int zs, int dx, int dy) -> unsigned {
+            const float qx = p0 + (float)(xs + dx * grp) * gp.step[0];
+            const float qy = p1 + (float)(ys + dy * grp) * gp.step[1];
+            const float qz = p2 + (float)zs * gp.step[2];
+            bool pos = false, neg = false;
+            if (fl < F) {
+                const float d = rnx * (qx - rfx) + rny * (qy - rfy) + rnz * (qz - rfz);
+                if (fabsf(d) > eps) {
+                    if (d >= 0.0f) pos = true;
+                    else neg = true;
+                }
+            }
+            const unsigned long long P = __ballot(pos), Q = __ballot(neg);
+            unsigned res = 0;
+            for (int g = 0; g < G; ++g)
+                if (!(((P >> (g * gs)) & gmask) && ((Q >> (g * gs)) & gmask))) res |= 1u << g;
+            return res;
+        };
+        const bool packed_ok = NF == 1 && n0 <= 64;
+        int ry = -1, rz = -1;
+        unsigned long long rvalid = 0, rin = 0;
+        // one point: every group tests it (NF == 1), or all faces of the
+        // larger-capacity kernel at once
+#define PROBE(xi, yi, zi)                                                                     \
+    (NF == 1 ? (packed(xi, yi, zi, 0, 0) & 1u) != 0u                                           \
+             : inside(p0 + (float)(xi) * gp.step[0], p1 + (float)(yi) * gp.step[1],           \
+                      p2 + (float)(zi) * gp.step[2]))
+#define ROW_PROBE(xi, yi, zi)                                                                 \
+    (packed_ok ? row_bit(xi, yi, zi) : PROBE(xi, yi, zi))
+        auto row_bit = [&](int xi, int yi, int zi) -> bool {
+            if (yi != ry || zi != rz) {
+                ry = yi;
+                rz = zi;
+                rvalid = 0;
+                rin = 0;
+            }
+            if (!((rvalid >> xi) & 1ull)) {
+                int xs = xi - (G - 1) / 2;
+                if (xs > n0 - G) xs = n0 - G;
+                if (xs < 0) xs = 0;
+                const unsigned res = packed(xs, yi, zi, 1, 0);
+                const int cnt = n0 - xs < G ? n0 - xs : G;
+                const unsigned long long wm = (cnt >= 64 ? ~0ull : ((1ull << cnt) - 1ull)) << xs;
+                rvalid |= wm;
+                rin |= ((unsigned long long)res << xs) & wm;
+            }
+            return (rin >> xi) & 1ull;
+        };
+
         // ---- 5. scanline search (src/dense.cpp:1486-1676) --------------------
+        // Pass 0 counts the points and keeps the runs (one per z-y line) in
+        // LDS.  A cell with more than RMAX runs scans again in pass 1 -- the
+        // state machine takes the same path -- and deposits each run as it is
+        // found, now that mass/N is known.
         int4 *runs = s_u;   // star neighbour records are no longer needed
         int nruns = 0, tot = 0;
-        int x_left = n0 / 2, x_right = n0 / 2, y_start = 0, first_x = 0;
-        bool run_over = false;
-        for (int zi = 0; zi < n2 && !run_over; zi++) {
-            bool border_found = false, z_step_done = false;
-            const float gz = p2 + (float)zi * gp.step[2];
-            for (int yi = y_start; yi < n1; yi++) {
-                const float gy = p1 + (float)yi * gp.step[1];
-                const bool x_in = inside(p0 + (float)x_left * gp.step[0], gy, gz);
-                int min_xi = n0 - 1, max_xi = 0;
-                for (int xi = x_left; xi >= 0 && xi < n0;) {
-                    if (inside(p0 + (float)xi * gp.step[0], gy, gz)) {
-                        if (xi < min_xi) min_xi = xi;
-                        if (xi > max_xi) max_xi = xi;
-                        if (x_in) xi--;
-                        else {
-                            x_left = xi;
-                            break;
-                        }
-                    } else {
-                        if (!x_in) xi++;
-                        else {
-                            x_left = xi;
-                            break;
-                        }
-                    }
-                }
-                for (int xi = x_right; xi >= 0 && xi < n0;) {
-                    if (inside(p0 + (float)xi * gp.step[0], gy, gz)) {
-                        if (xi < min_xi) min_xi = xi;
-                        if (xi > max_xi) max_xi = xi;
-                        if (x_in) xi++;
-                        else {
-                            x_right = xi;
-                            break;
-                        }
-                    } else {
-                        if (!x_in) xi--;
-                        else {
-                            x_right = xi;
-                            break;
+        float mpp = 0.0f;
+        for (int pass = 0; pass < 2; ++pass) {
+            if (pass == 1) {
+                if (nruns <= RMAX || tot == 0) break;
+                mpp = gp.mass / (float)tot;   // src/dense.cpp:1692
+            }
+            int x_left = n0 / 2, x_right = n0 / 2, y_start = 0, first_x = 0;
+            for (int zi = 0; zi < n2; zi++) {
+                bool border_found = false, z_step_done = false;
+                for (int yi = y_start; yi < n1; yi++) {
+                    // the verdict at x0 is reused by both walks (PtInCell is pure)
+                    const int x0 = x_left;
+                    const bool x_in = ROW_PROBE(x0, yi, zi);
+                    int min_xi = n0 - 1, max_xi = 0;
+                    for (int xi = x_left; xi >= 0 && xi < n0;) {
+                        if (xi == x0 ? x_in : ROW_PROBE(xi, yi, zi)) {
+                            if (xi < min_xi) min_xi = xi;
+                            if (xi > max_xi) max_xi = xi;
+                            if (x_in) xi--;
+                            else {
+                                x_left = xi;
+                                break;
+                            }
+                        } else {
+                            if (!x_in) xi++;
+                            else {
+                                x_left = xi;
+                                break;
+                            }
                         }
                     }
-                }
-                int first_y = y_start;
-                if (min_xi <= max_xi) {
-                    if (nruns >= RMAX) {
-                        run_over = true;
-                        break;
+                    for (int xi = x_right; xi >= 0 && xi < n0;) {
+                        if (xi == x0 ? x_in : ROW_PROBE(xi, yi, zi)) {
+                            if (xi < min_xi) min_xi = xi;
+                            if (xi > max_xi) max_xi = xi;
+                            if (x_in) xi++;
+                            else {
+                                x_right = xi;
+                                break;
+                            }
+                        } else {
+                            if (!x_in) xi--;
+                            else {
+                                x_right = xi;
+                                break;
+                            }
+                        }
                     }
-                    if (lane == 0) runs[nruns] = make_int4(zi, yi, min_xi, tot);
-                    nruns++;
-                    tot += (max_xi - min_xi + 1);
-                    if (yi == y_start) first_x = (min_xi + max_xi) / 2;
-                    if (!border_found) {
-                        first_y = yi;
-                        border_found = true;
+                    int first_y = y_start;
+                    if (min_xi <= max_xi) {
+                        if (pass == 0) {
+                            if (nruns < RMAX && lane == 0)
+                                runs[nruns] = make_int4(zi, yi, min_xi, tot);
+                            nruns++;
+                            tot += (max_xi - min_xi + 1);
+                        } else {
+                            const int len = max_xi - min_xi + 1;
+                            for (int b0 = 0; b0 < len; b0 += 64)
+                                if (b0 + lane < len)
+                                    deposit_point(cmi0 + min_xi + b0 + lane, cmi1 + yi, cmi2 + zi,
+                                                  mpp, v, gp, o, rt, self);
+                        }
+                        if (yi == y_start) first_x = (min_xi + max_xi) / 2;
+                        if (!border_found) {
+                            first_y = yi;
+                            border_found = true;
+                        }
+                    } else if (border_found) {
+                        z_step_done = true;
                     }
-                } else if (border_found) {
-                    z_step_done = true;
-                }
-                if ((yi == n1 - 1 || z_step_done) && zi + 1 < n2) {
-                    const float gz1 = p2 + (float)(zi + 1) * gp.step[2];
-                    const float gx1 = p0 + (float)first_x * gp.step[0];
-                    int yj;
-                    for (yj = first_y; yj > 0; yj--) {
-                        if (!inside(gx1, p1 + (float)yj * gp.step[1], gz1)) break;
+                    if ((yi == n1 - 1 || z_step_done) && zi + 1 < n2) {
+                        int yj = first_y;
+                        if (NF == 1 && G > 1) {
+                            // G points down the column at once
+                            bool stop = false;
+                            while (yj > 0 && !stop) {
+                                const unsigned res = packed(first_x, yj, zi + 1, 0, -1);
+                                for (int g = 0; g < G && yj > 0; ++g) {
+                                    if (!((res >> g) & 1u)) {
+                                        stop = true;
+                                        break;
+                                    }
+                                    --yj;
+                                }
+                            }
+                        } else {
+                            for (; yj > 0; yj--) {
+                                if (!PROBE(first_x, yj, zi + 1)) break;
+                            }
+                        }
+                        y_start = yj;
                     }
-                    y_start = yj;
+                    if (z_step_done) break;
                 }
-                if (z_step_done) break;
             }
         }
-        if (run_over) goto defer;
+#undef PROBE
+#undef ROW_PROBE
         __syncthreads();
 
         // ---- 6. deposit -------------------------------------------------------
-        if (tot > 0) {
-            const float mpp = gp.mass / (float)tot;   // src/dense.cpp:1692
+        if (tot > 0 && nruns > RMAX) {
+            if (lane == 0) {   // deposited by pass 1
+                o.status[v] = ST_SCAN;
+                o.npts[v] = tot;
+            }
+        } else if (tot > 0) {
+            mpp = gp.mass / (float)tot;   // src/dense.cpp:1692
             for (int base = 0; base < tot; base += 64) {
                 const int p = base + lane;
                 int r = 0;
@@ -485,7 +592,7 @@ __device__ void cell_ref_body(const int v, const BlockIn &in, const GridP &gp,
                 if (p < tot) {
                     const int4 rr = runs[r];
                     deposit_point(cmi0 + rr.z + (p - rr.w), cmi1 + rr.y, cmi2 + rr.x, mpp, v,
-                                  gp, o);
+                                  gp, o, rt, self);
                 }
             }
             if (lane == 0) {
@@ -502,7 +609,7 @@ __device__ void cell_ref_body(const int v, const BlockIn &in, const GridP &gp,
                 for (int k = 1; k < 8; ++k)
                     if (lane == k) m = m8[k];
                 deposit_point(mn[0] + (lane & 1), mn[1] + ((lane >> 1) & 1), mn[2] + (lane >> 2),
-                              m, v, gp, o);
+                              m, v, gp, o, rt, self);
             }
             if (lane == 0) {
                 o.status[v] = ST_CIC;
@@ -534,10 +641,11 @@ __global__ __launch_bounds__(64) void k_cells_ref(BlockIn in, GridP gp, CellOut 
     __shared__ int s_ns[64 * NS * 4];
     __shared__ float4 s_cc[64 * NS];
     __shared__ int4 s_u[(64 * NS > RMAX) ? 64 * NS : RMAX];
+    __shared__ int s_h[hash_ints<NS>()];
     const int k = xcd_item((int)blockIdx.x, in.ncells);
     if (k < 0) return;
     const int v = in.order ? in.order[k] : k;
-    cell_ref_body<NS, NF, RMAX>(v, in, gp, o, want_density, true, s_v, s_ns, s_cc, s_u);
+    cell_ref_body<NS, NF, RMAX>(v, in, gp, o, want_density, true, s_v, s_ns, s_cc, s_u, s_h);
 }
 
 // large-capacity pass over the cells the fast kernel deferred; the grid is
@@ -550,13 +658,90 @@ __global__ __launch_bounds__(64) void k_cells_ref_slow(BlockIn in, GridP gp, Cel
     __shared__ int s_ns[64 * NS * 4];
     __shared__ float4 s_cc[64 * NS];
     __shared__ int4 s_u[(64 * NS > RMAX) ? 64 * NS : RMAX];
+    __shared__ int s_h[hash_ints<NS>()];
     unsigned n = o.ctrs[C_SLOW];
     if (n > o.slow_cap) n = o.slow_cap;
     for (unsigned k = blockIdx.x; k < n; k += gridDim.x) {
         const int v = o.slow[k];
-        cell_ref_body<NS, NF, RMAX>(v, in, gp, o, want_density, false, s_v, s_ns, s_cc, s_u);
+        cell_ref_body<NS, NF, RMAX>(v, in, gp, o, want_density, false, s_v, s_ns, s_cc, s_u, s_h);
         __syncthreads();
     }
+}
+
+// All blocks of a run in one persistent launch (at least 6 waves per SIMD:
+// the per-block descriptors read from memory otherwise push the register
+// count to 4 waves; measured 30 vs 34.5 ms on 64 Plummer blocks): every wave takes chunks of
+// cells (one block each, consecutive in the block's Morton order) from a
+// global queue until the queue is empty.  Chunks are listed largest block
+// first, with chunk sizes by block size class (tess2 autotuner).
+template <int NS, int NF, int RMAX>
+__global__ __launch_bounds__(64, 6) void k_cells_ref_batched(const BlockIn *__restrict__ ins,
+                                                          const GridP *__restrict__ gps,
+                                                          const CellOut *__restrict__ outs,
+                                                          const Router *__restrict__ rt,
+                                                          const Chunk *__restrict__ chunks,
+                                                          int nchunks, unsigned *queue,
+                                                          int want_density)
+{
+    __shared__ int4 s_v[64 * NS];
+    __shared__ int s_ns[64 * NS * 4];
+    __shared__ float4 s_cc[64 * NS];
+    __shared__ int4 s_u[(64 * NS > RMAX) ? 64 * NS : RMAX];
+    __shared__ int s_h[hash_ints<NS>()];
+    const int lane = lane_id();
+    for (;;) {
+        int c = 0;
+        if (lane == 0) c = (int)atomicAdd(queue, 1u);
+        c = __builtin_amdgcn_readfirstlane(c);
+        if (c >= nchunks) break;
+        const Chunk ch = chunks[c];
+        // read-only descriptors (scalar loads; rt->gps/outs alias them for routing)
+        const BlockIn in = ins[ch.block];
+        const GridP gp = gps[ch.block];
+        const CellOut o = outs[ch.block];
+        for (int j = 0; j < ch.count; ++j) {
+            const int k = ch.start + j;
+            const int v = in.order ? in.order[k] : k;
+            cell_ref_body<NS, NF, RMAX>(v, in, gp, o, want_density, true, s_v, s_ns, s_cc, s_u,
+                                        s_h, rt, ch.block);
+            __syncthreads();
+        }
+    }
+}
+
+template <int NS, int NF, int RMAX>
+__global__ __launch_bounds__(64) void k_cells_ref_slow_batched(const BlockIn *__restrict__ ins,
+                                                               const Router *__restrict__ rt,
+                                                               int want_density)
+{
+    __shared__ int4 s_v[64 * NS];
+    __shared__ int s_ns[64 * NS * 4];
+    __shared__ float4 s_cc[64 * NS];
+    __shared__ int4 s_u[(64 * NS > RMAX) ? 64 * NS : RMAX];
+    __shared__ int s_h[hash_ints<NS>()];
+    const int nb = rt->nblocks;
+    for (int b = 0; b < nb; ++b) {
+        const CellOut &o = rt->outs[b];
+        unsigned n = o.ctrs[C_SLOW];
+        if (n > o.slow_cap) n = o.slow_cap;
+        for (unsigned k = blockIdx.x; k < n; k += gridDim.x) {
+            cell_ref_body<NS, NF, RMAX>(o.slow[k], ins[b], rt->gps[b], o, want_density, false, s_v,
+                                        s_ns, s_cc, s_u, s_h, rt, b);
+            __syncthreads();
+        }
+    }
+}
+
+void launch_cells_ref_batched(const BlockIn *ins, const GridP *gps, const CellOut *outs,
+                              const Router *rt, const Chunk *chunks, int nchunks,
+                              unsigned *queue, int waves, int want_density, hipStream_t s)
+{
+    launch_fill_u32(queue, 1, 0u, s);
+    if (nchunks > 0)
+        hipLaunchKernelGGL((k_cells_ref_batched<1, 1, 64>), dim3(waves), dim3(64), 0, s, ins, gps,
+                           outs, rt, chunks, nchunks, queue, want_density);
+    hipLaunchKernelGGL((k_cells_ref_slow_batched<8, 2, 512>), dim3(1024), dim3(64), 0, s, ins, rt,
+                       want_density);
 }
 
 void launch_cells_ref(const BlockIn &in, const GridP &gp, const CellOut &o, int want_density,
@@ -571,14 +756,14 @@ void launch_cells_ref_slow(const BlockIn &in, const GridP &gp, const CellOut &o,
                            int want_density, hipStream_t s)
 {
     if (in.ncells <= 0) return;
-    hipLaunchKernelGGL((k_cells_ref_slow<8, 2, 2048>), dim3(512), dim3(64), 0, s, in, gp,
+    hipLaunchKernelGGL((k_cells_ref_slow<8, 2, 512>), dim3(1024), dim3(64), 0, s, in, gp,
                        o, want_density);
 }
 
 // ---------------------------------------------------------------------------
 // DENSE_CIC estimator: IterateCellsCic, src/dense.cpp:486-562
 // ---------------------------------------------------------------------------
-__global__ void k_cic(BlockIn in, GridP gp, CellOut o)
+__global__ void k_cic(BlockIn in, GridP gp, CellOut o, const Router *rt, int self)
 {
     const int v = blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= in.ncells) return;
@@ -588,65 +773,17 @@ __global__ void k_cic(BlockIn in, GridP gp, CellOut o)
     cic_weights(p.x, p.y, p.z, gp.mass, gp, mn, m8);
     for (int k = 0; k < 8; ++k)
         deposit_point(mn[0] + (k & 1), mn[1] + ((k >> 1) & 1), mn[2] + (k >> 2), m8[k], v,
-                      gp, o);
+                      gp, o, rt, self);
     o.status[v] = ST_CIC;
     o.npts[v] = 8;
 }
 
-void launch_cic(const BlockIn &in, const GridP &gp, const CellOut &o, hipStream_t s)
+void launch_cic(const BlockIn &in, const GridP &gp, const CellOut &o, const Router *rt_dev,
+                int self, hipStream_t s)
 {
     if (in.ncells <= 0) return;
-    hipLaunchKernelGGL(k_cic, dim3((in.ncells + 255) / 256), dim3(256), 0, s, in, gp, o);
-}
-
-// ---------------------------------------------------------------------------
-// ordered fold of shared grid points: contributions in ascending cell id,
-// density = (float)((double)density + mass/div) each (src/dense.cpp:290)
-// ---------------------------------------------------------------------------
-__global__ void k_finalize(CellOut o, float div)
-{
-    const unsigned n = o.ctrs[C_OVF];
-    if (n > o.ovf_cap) return;   // list overflowed: the host reruns with more room
-    for (unsigned e = blockIdx.x * blockDim.x + threadIdx.x; e < n;
-         e += gridDim.x * blockDim.x) {
-        const int4 r = o.ovf[e];
-        const int lidx = r.x;
-        if (o.head[lidx] != (int)e) continue;
-        const unsigned cnt = o.count[lidx];
-        const uint2 fst = o.first[lidx];
-        float rho = 0.0f;
-        int prev = -1;
-        for (unsigned k = 0; k < cnt; ++k) {
-            // next contribution: smallest cell id above prev
-            int best = 0x7fffffff;
-            float bm = 0.0f;
-            if ((int)fst.x > prev && (int)fst.x < best) {
-                best = (int)fst.x;
-                bm = __uint_as_float(fst.y);
-            }
-            int q = (int)e;
-            for (unsigned j = 0; j + 1 < cnt; ++j) {
-                const int4 rq = o.ovf[q];
-                if (rq.y > prev && rq.y < best) {
-                    best = rq.y;
-                    bm = __int_as_float(rq.z);
-                }
-                q = rq.w;
-            }
-            rho = (float)((double)rho + (double)bm / (double)div);
-            prev = best;
-        }
-        o.density[lidx] = rho;
-    }
-}
-
-void launch_finalize(const CellOut &o, const GridP &gp, hipStream_t s)
-{
-    const unsigned cap = o.ovf_cap;
-    unsigned blocks = (cap + 255) / 256;
-    if (blocks > 4096) blocks = 4096;
-    if (blocks == 0) blocks = 1;
-    hipLaunchKernelGGL(k_finalize, dim3(blocks), dim3(256), 0, s, o, gp.div);
+    hipLaunchKernelGGL(k_cic, dim3((in.ncells + 255) / 256), dim3(256), 0, s, in, gp, o, rt_dev,
+                       self);
 }
 
 }  // namespace t2

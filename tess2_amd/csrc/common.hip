@@ -2,6 +2,8 @@
 #include "kernels.hpp"
 #include "geom.hpp"
 
+#include <algorithm>
+
 namespace t2 {
 
 // fill_circumcenters(): one lane per tet, src/volume.cpp:6-11.  The four
@@ -28,6 +30,53 @@ void launch_circumcenters(const float4 *pts, const int4 *tets, int ntets, float4
                        ntets, cc);
 }
 
+// circumcenters of every staged block, block = blockIdx.y
+__global__ void k_circumcenters_blocks(const BlockIn *__restrict__ ins)
+{
+    const BlockIn &in = ins[blockIdx.y];
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < in.ntets;
+         t += gridDim.x * blockDim.x) {
+        const int4 vv = in.tets[2 * t];
+        const float4 a = in.pts[vv.x], b = in.pts[vv.y], c = in.pts[vv.z], d = in.pts[vv.w];
+        float x, y, z;
+        circumcenter(a.x, a.y, a.z, b.x, b.y, b.z, c.x, c.y, c.z, d.x, d.y, d.z, x, y, z);
+        const float e = circumcenter_err(a.x, a.y, a.z, b.x, b.y, b.z, c.x, c.y, c.z, d.x, d.y,
+                                         d.z, x, y, z);
+        const_cast<float4 *>(in.cc)[t] = make_float4(x, y, z, e);   // read-only downstream
+    }
+}
+
+// zero the density and count grids of every block, block = blockIdx.y
+__global__ void k_clear_grids(const CellOut *__restrict__ outs)
+{
+    const CellOut &o = outs[blockIdx.y];
+    const unsigned long long n = o.ngrid, n4 = n / 4;
+    const unsigned long long stride = (unsigned long long)gridDim.x * blockDim.x;
+    const unsigned long long i0 = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint4 z = make_uint4(0u, 0u, 0u, 0u);
+    for (unsigned long long i = i0; i < n4; i += stride) {
+        reinterpret_cast<uint4 *>(o.density)[i] = z;
+        reinterpret_cast<uint4 *>(o.count)[i] = z;
+    }
+    if (i0 < n - n4 * 4) {
+        o.density[n4 * 4 + i0] = 0.0f;
+        o.count[n4 * 4 + i0] = 0u;
+    }
+}
+
+void launch_block_prep(const BlockIn *ins, const CellOut *outs, int nb, int max_tets,
+                       unsigned long long max_grid, bool circumcenters, hipStream_t s)
+{
+    if (nb <= 0) return;
+    unsigned bg = (unsigned)std::min<unsigned long long>(2048, (max_grid / 4 + 255) / 256);
+    if (bg == 0) bg = 1;
+    hipLaunchKernelGGL(k_clear_grids, dim3(bg, nb), dim3(256), 0, s, outs);
+    if (circumcenters && max_tets > 0) {
+        const unsigned bt = (unsigned)std::min(16384, (max_tets + 255) / 256);
+        hipLaunchKernelGGL(k_circumcenters_blocks, dim3(bt, nb), dim3(256), 0, s, ins);
+    }
+}
+
 __global__ void k_fill_u32(uint4 *__restrict__ p4, size_t n4, unsigned *__restrict__ tail,
                            size_t ntail, unsigned v)
 {
@@ -52,42 +101,40 @@ void launch_fill_u32(unsigned *p, size_t n, unsigned v, hipStream_t s)
                        p + n4 * 4, ntail, v);
 }
 
-// status histogram (5 codes) and total points of deposited cells
-__global__ void k_cell_stats(const int *__restrict__ status, const int *__restrict__ npts,
-                             int n, unsigned long long *out6)
+// per block (blockIdx.y): status histogram (5 codes) and total points of
+// deposited cells
+__global__ void k_cell_stats(const CellOut *__restrict__ outs)
 {
+    const CellOut &o = outs[blockIdx.y];
     unsigned long long c[6] = {0, 0, 0, 0, 0, 0};
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        const int s = status[i];
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < o.ncells;
+         i += gridDim.x * blockDim.x) {
+        const int s = o.status[i];
         if (s >= 0 && s < 5) c[s]++;
-        if (s == 0 || s == 1) c[5] += (unsigned long long)npts[i];
+        if (s == 0 || s == 1) c[5] += (unsigned long long)o.npts[i];
     }
     for (int k = 0; k < 6; ++k) {
         unsigned long long x = c[k];
         for (int off = 32; off >= 1; off >>= 1) x += __shfl_xor(x, off);
-        if ((threadIdx.x & 63) == 0 && x) atomicAdd(&out6[k], x);
+        if ((threadIdx.x & 63) == 0 && x) atomicAdd(&o.st->cstat[k], x);
     }
 }
 
-void launch_cell_stats(const int *status, const int *npts, int ncells,
-                       unsigned long long *out6, hipStream_t s)
-{
-    if (ncells <= 0) return;
-    int blocks = (ncells + 255) / 256;
-    if (blocks > 1024) blocks = 1024;
-    hipLaunchKernelGGL(k_cell_stats, dim3(blocks), dim3(256), 0, s, status, npts, ncells,
-                       out6);
-}
+// per block (blockIdx.y): sum (double) and max of the density grid.  The
+// sum is reduced in a fixed order (lanes, waves, then workgroups in
+// k_grid_final) so that it is the same on every run.
+constexpr int GRID_PARTS = 256;
 
-// sum (double) and max of the density grid
-__global__ void k_grid_stats(const float *__restrict__ d, size_t n, double *sum,
-                             unsigned *maxbits)
+__global__ __launch_bounds__(256) void k_grid_stats(const CellOut *__restrict__ outs,
+                                                    double *__restrict__ part)
 {
+    __shared__ double s_acc[4];
+    const CellOut &o = outs[blockIdx.y];
     double acc = 0.0;
     float mx = 0.0f;
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (size_t)gridDim.x * blockDim.x) {
-        const float x = d[i];
+    for (unsigned long long i = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;
+         i < o.ngrid; i += (unsigned long long)gridDim.x * blockDim.x) {
+        const float x = o.density[i];
         acc += (double)x;
         mx = fmaxf(mx, x);
     }
@@ -96,19 +143,36 @@ __global__ void k_grid_stats(const float *__restrict__ d, size_t n, double *sum,
         mx = fmaxf(mx, __shfl_xor(mx, off));
     }
     if ((threadIdx.x & 63) == 0) {
-        atomicAdd(sum, acc);
-        atomicMax(maxbits, __float_as_uint(mx));   // non-negative floats order as uints
+        s_acc[threadIdx.x >> 6] = acc;
+        if (mx != 0.0f) atomicMax(&o.st->gmax, __float_as_uint(mx));   // order-free
     }
+    __syncthreads();
+    if (threadIdx.x == 0)
+        part[blockIdx.y * GRID_PARTS + blockIdx.x] = ((s_acc[0] + s_acc[1]) + s_acc[2]) + s_acc[3];
 }
 
-void launch_grid_stats(const float *density, size_t n, double *sum, unsigned *maxbits,
-                       hipStream_t s)
+__global__ void k_grid_final(const CellOut *__restrict__ outs, const double *__restrict__ part,
+                             int nparts)
 {
-    if (n == 0) return;
-    size_t blocks = (n + 255) / 256;
-    if (blocks > 1024) blocks = 1024;
-    hipLaunchKernelGGL(k_grid_stats, dim3((unsigned)blocks), dim3(256), 0, s, density, n, sum,
-                       maxbits);
+    if (threadIdx.x != 0) return;
+    double acc = 0.0;
+    for (int k = 0; k < nparts; ++k) acc += part[blockIdx.x * GRID_PARTS + k];
+    outs[blockIdx.x].st->gsum = acc;
+}
+
+void launch_stats(const CellOut *outs, double *part, int nb, int max_cells,
+                  unsigned long long max_grid, hipStream_t s)
+{
+    if (nb <= 0) return;
+    // about 16 items per lane, at most 256 workgroups per block
+    unsigned bc = (unsigned)std::min<long long>(256, ((long long)max_cells + 4095) / 4096);
+    unsigned bg = (unsigned)std::min<unsigned long long>(GRID_PARTS, (max_grid + 4095) / 4096);
+    if (bc == 0) bc = 1;
+    if (bg == 0) bg = 1;
+    hipLaunchKernelGGL(k_cell_stats, dim3(bc, nb), dim3(256), 0, s, outs);
+    hipLaunchKernelGGL(k_grid_stats, dim3(bg, nb), dim3(256), 0, s, outs, part);
+    hipLaunchKernelGGL(k_grid_final, dim3(nb), dim3(64), 0, s, outs, (const double *)part,
+                       (int)bg);
 }
 
 }  // namespace t2

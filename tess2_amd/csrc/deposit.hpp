@@ -9,39 +9,89 @@
 namespace t2 {
 
 // ---------------------------------------------------------------------------
-// deposit one grid point (src/dense.cpp:276-305)
+// ordered-accumulation protocol on one block grid point: the first
+// contribution writes the density; later ones join the point's list.  key
+// orders the fold: a block's own cells by cell id (phase 0), then points
+// received from other blocks by (source rank, cell) (phase 1, bit 31).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void push_point(long long lidx, float m, unsigned key, float div,
+                                           const CellOut &o)
+{
+    const unsigned c = atomicAdd(&o.count[lidx], 1u);
+    if (c == 0u) {
+        o.first[lidx] = make_uint2(key, __float_as_uint(m));
+        o.density[lidx] = (float)((double)m / (double)div);
+    }
+    // list entries: one atomic per wave and block counter (the active lanes
+    // of a wave may deposit into several blocks)
+    const int lane = threadIdx.x & 63;
+    unsigned *ctr = &o.ctrs[C_OVF];
+    const unsigned long long cp = reinterpret_cast<unsigned long long>(ctr);
+    bool pend = c != 0u;
+    unsigned e = 0;
+    for (;;) {
+        const unsigned long long act = __ballot(pend);
+        if (!act) break;
+        const int leader = __ffsll((unsigned long long)act) - 1;
+        const unsigned lo = __builtin_amdgcn_readlane((unsigned)cp, leader);
+        const unsigned hi = __builtin_amdgcn_readlane((unsigned)(cp >> 32), leader);
+        const bool same = pend && (unsigned)cp == lo && (unsigned)(cp >> 32) == hi;
+        const unsigned long long sm = __ballot(same);
+        unsigned b0 = 0;
+        if (lane == leader) b0 = atomicAdd(ctr, (unsigned)__popcll(sm));
+        b0 = __shfl(b0, leader);
+        if (same) {
+            e = b0 + (unsigned)__popcll(sm & ((1ull << lane) - 1ull));
+            pend = false;
+        }
+    }
+    if (c != 0u && e < o.ovf_cap) {
+        const int prev = atomicExch(&o.head[lidx], (int)e);
+        o.ovf[e] = make_int4((int)lidx, (int)key, __float_as_int(m), prev);
+    }
+}
+
+__device__ __forceinline__ long long block_lidx(int gx, int gy, int gz, const GridP &gp)
+{
+    const int lx = gx - gp.bmin_idx[0], ly = gy - gp.bmin_idx[1], lz = gz - gp.bmin_idx[2];
+    if (lx < 0 || ly < 0 || lx >= gp.bnum_idx[0] || ly >= gp.bnum_idx[1]) return -1;
+    if (gp.project) return (long long)ly * gp.bnum_idx[0] + lx;
+    if (lz < 0 || lz >= gp.bnum_idx[2]) return -1;
+    return ((long long)lz * gp.bnum_idx[1] + ly) * gp.bnum_idx[0] + lx;
+}
+
+// ---------------------------------------------------------------------------
+// deposit one grid point (src/dense.cpp:276-305).  In the block (inside its
+// bounds and its grid): the block's own grid.  Otherwise the point goes to
+// the block whose grid holds its global index, as received mass
+// (src/dense.cpp:298-305 -> recvd_pts :166-202), or is dropped and counted.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ void deposit_point(int gx, int gy, int gz, float m, int cell,
-                                              const GridP &gp, const CellOut &o)
+                                              const GridP &gp, const CellOut &o,
+                                              const Router *rt = nullptr, int self = 0)
 {
     const float px = idx2phys1(gx, gp.step[0], gp.gmin[0]);
     const float py = idx2phys1(gy, gp.step[1], gp.gmin[1]);
     const float pz = idx2phys1(gz, gp.step[2], gp.gmin[2]);
-    if (!(px >= gp.bmin[0] && px <= gp.bmax[0] && py >= gp.bmin[1] && py <= gp.bmax[1] &&
-          pz >= gp.bmin[2] && pz <= gp.bmax[2])) {
-        atomicAdd(&o.ctrs[C_OFFBLOCK], 1u);
+    const long long lidx = block_lidx(gx, gy, gz, gp);
+    if (lidx >= 0 && px >= gp.bmin[0] && px <= gp.bmax[0] && py >= gp.bmin[1] &&
+        py <= gp.bmax[1] && pz >= gp.bmin[2] && pz <= gp.bmax[2]) {
+        push_point(lidx, m, (unsigned)cell, gp.div, o);
         return;
     }
-    const int lx = gx - gp.bmin_idx[0], ly = gy - gp.bmin_idx[1], lz = gz - gp.bmin_idx[2];
-    if (lx < 0 || ly < 0 || lx >= gp.bnum_idx[0] || ly >= gp.bnum_idx[1] ||
-        (!gp.project && (lz < 0 || lz >= gp.bnum_idx[2]))) {
-        atomicAdd(&o.ctrs[C_OFFGRID], 1u);
-        return;
-    }
-    const long long lidx =
-        gp.project ? (long long)ly * gp.bnum_idx[0] + lx
-                   : ((long long)lz * gp.bnum_idx[1] + ly) * gp.bnum_idx[0] + lx;
-    const unsigned c = atomicAdd(&o.count[lidx], 1u);
-    if (c == 0u) {
-        o.first[lidx] = make_uint2((unsigned)cell, __float_as_uint(m));
-        o.density[lidx] = (float)((double)m / (double)gp.div);
-    } else {
-        const unsigned e = atomicAdd(&o.ctrs[C_OVF], 1u);
-        if (e < o.ovf_cap) {
-            const int prev = atomicExch(&o.head[lidx], (int)e);
-            o.ovf[e] = make_int4((int)lidx, cell, __float_as_int(m), prev);
+    if (rt) {
+        for (int j = 0; j < rt->nblocks; ++j) {
+            if (j == self) continue;
+            const long long lj = block_lidx(gx, gy, gz, rt->gps[j]);
+            if (lj >= 0) {
+                const unsigned key = 0x80000000u | ((unsigned)self << 24) | (unsigned)cell;
+                push_point(lj, m, key, rt->gps[j].div, rt->outs[j]);
+                atomicAdd(&o.ctrs[C_ROUTED], 1u);
+                return;
+            }
         }
     }
+    atomicAdd(&o.ctrs[C_OFFBLOCK], 1u);
 }
 
 // DistributeScalarCIC(): src/dense.cpp:1787-1872, corner k in z,y,x order.

@@ -320,7 +320,7 @@ template <int NS>
 __device__ void exact_status_body(const int v, const BlockIn &in, const GridP &gp, int *status,
                                   const int prec, unsigned *ctrs, int *slow,
                                   const unsigned slow_cap, const bool can_defer, int4 *s_v,
-                                  int4 *s_n)
+                                  int4 *s_n, int *s_h)
 {
     constexpr int SMAX = 64 * NS;
     const int lane = threadIdx.x & 63;
@@ -329,13 +329,18 @@ __device__ void exact_status_body(const int v, const BlockIn &in, const GridP &g
         if (lane == 0) status[v] = EX_INCOMPLETE;
         return;
     }
-    int sid[NS];
-#pragma unroll
-    for (int k = 0; k < NS; ++k) sid[k] = -1;
+    // the star as a set (the decision below does not depend on its order):
+    // tet ids enter an LDS hash table, the lane whose insert creates an entry
+    // gives the tet the next slot
+    constexpr int HS = 2 * SMAX;
+    int *hk = s_h, *s_tid = s_h + HS;
+    for (int i = lane; i < HS; i += 64) hk[i] = -1;
+    __syncthreads();
     if (lane == 0) {
         s_v[0] = in.tets[2 * vt];
         s_n[0] = in.tets[2 * vt + 1];
-        sid[0] = vt;
+        (void)hash_find<HS>(hk, vt);
+        s_tid[0] = vt;
     }
     int S = 1, f0 = 0, f1 = 1;
     bool hull = false, over = false;
@@ -361,40 +366,29 @@ __device__ void exact_status_body(const int v, const BlockIn &in, const GridP &g
                 hull = true;
                 break;
             }
-            unsigned long long pend = __ballot(valid);
-            while (pend) {
-                const int leader = __ffsll((unsigned long long)pend) - 1;
-                const int val = __builtin_amdgcn_readlane(cand, leader);
-                const unsigned long long same = __ballot(valid && cand == val);
-                bool found = false;
-#pragma unroll
-                for (int k = 0; k < NS; ++k)
-                    if (__ballot(sid[k] == val)) found = true;
-                if (!found) {
-                    if (S < SMAX) {
-#pragma unroll
-                        for (int k = 0; k < NS; ++k)
-                            if (k == (S >> 6) && lane == (S & 63)) sid[k] = val;
-                    }
-                    S++;
-                }
-                pend &= ~same;
+            bool ins = false;
+            const int e = valid ? hash_find<HS>(hk, cand, ins) : 0;
+            if (__ballot(valid && e < 0)) {   // table full: the star is too large
+                over = true;
+                break;
+            }
+            const unsigned long long L = __ballot(ins);
+            if (ins) {
+                const int slot = S + __popcll(L & ((1ull << lane) - 1ull));
+                if (slot < SMAX) s_tid[slot] = cand;
+            }
+            S += __popcll(L);
+            if (S > SMAX) {
+                over = true;
+                break;
             }
         }
-        if (hull) break;
-        if (S > SMAX) {
-            over = true;
-            break;
-        }
+        if (hull || over) break;
+        __syncthreads();
         for (int b = f1; b < S; b += 64) {
             const int s = b + lane;
-            int id = -1;
-#pragma unroll
-            for (int k = 0; k < NS; ++k) {
-                const int t = __shfl(sid[k], s & 63);
-                if ((s >> 6) == k) id = t;
-            }
             if (s < S) {
+                const int id = s_tid[s];
                 s_v[s] = in.tets[2 * id];
                 s_n[s] = in.tets[2 * id + 1];
             }
@@ -437,7 +431,7 @@ __device__ void exact_status_body(const int v, const BlockIn &in, const GridP &g
         for (int k = 0; k < NS; ++k) {
             const int s = k * 64 + lane;
             if (s < S) {
-                const float4 c = in.cc[sid[k]];
+                const float4 c = in.cc[s_tid[s]];
                 mn[0] = fminf(mn[0], c.x);
                 mn[1] = fminf(mn[1], c.y);
                 mn[2] = fminf(mn[2], c.z);
@@ -514,10 +508,11 @@ __global__ __launch_bounds__(64) void k_exact_status(BlockIn in, GridP gp, int *
 {
     __shared__ int4 s_v[64];
     __shared__ int4 s_n[64];
+    __shared__ int s_h[3 * 64];
     const int k = xcd_item((int)blockIdx.x, in.ncells);
     if (k < 0) return;
     const int v = in.order ? in.order[k] : k;
-    exact_status_body<1>(v, in, gp, status, prec, ctrs, slow, slow_cap, true, s_v, s_n);
+    exact_status_body<1>(v, in, gp, status, prec, ctrs, slow, slow_cap, true, s_v, s_n, s_h);
 }
 
 __global__ __launch_bounds__(64) void k_exact_status_slow(BlockIn in, GridP gp, int *status,
@@ -526,11 +521,12 @@ __global__ __launch_bounds__(64) void k_exact_status_slow(BlockIn in, GridP gp, 
 {
     __shared__ int4 s_v[512];
     __shared__ int4 s_n[512];
+    __shared__ int s_h[3 * 512];
     unsigned n = ctrs[C_SLOW];
     if (n > slow_cap) n = slow_cap;
     for (unsigned k = blockIdx.x; k < n; k += gridDim.x) {
         exact_status_body<8>(slow[k], in, gp, status, prec, ctrs, slow, slow_cap, false, s_v,
-                             s_n);
+                             s_n, s_h);
         __syncthreads();
     }
 }

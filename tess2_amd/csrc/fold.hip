@@ -1,0 +1,371 @@
+// fold.hip -- the ordered fold of shared grid points.
+//
+// The reference adds every contribution to a grid point in its own order:
+// a block's cells in ascending id (IterateCells, src/dense.cpp:220-313),
+// then points received from other blocks (recvd_pts, src/dense.cpp:166-202),
+// each as density = (float)((double)density + mass / div)
+// (src/dense.cpp:290).  The cell kernels record the first contribution of a
+// point next to the point and append later ones to a shared list; this pass
+// replays the additions of every point that got more than one in ascending
+// key order (key = cell id, or the received-point key above 2^31):
+//   1. gather: each shared point (found through its list head) reserves a
+//      contiguous segment, sized by its contribution count, and is filed in
+//      one of three classes by that count;
+//   2. scatter: list entries go to their point's segment;
+//   3. fold: sort each segment by key and add in order -- in registers (at
+//      most 16 contributions, one lane per point), in LDS (at most 2048, one
+//      wave per point), or by chunk sorts in LDS plus merge passes in global
+//      memory (larger, one 1024-lane workgroup per point).
+// The additions of one point are a chain of dependent float roundings, so
+// each runs on a single lane; only the division by div is done in parallel.
+#include "kernels.hpp"
+
+namespace t2 {
+
+__device__ __forceinline__ bool fold_overflowed(const CellOut &o)
+{
+    return o.ctrs[C_OVF] > o.ovf_cap;   // the host reruns with more room
+}
+
+// class list and capacity by a run-time class (selects, not an indexed load:
+// an indexed member of the local descriptor would live in scratch)
+__device__ __forceinline__ int4 *fold_list(const CellOut &o, int cls)
+{
+    return cls == 0 ? o.fold[0] : (cls == 1 ? o.fold[1] : o.fold[2]);
+}
+__device__ __forceinline__ unsigned fold_capacity(const CellOut &o, int cls)
+{
+    return cls == 0 ? o.fold_cap[0] : (cls == 1 ? o.fold_cap[1] : o.fold_cap[2]);
+}
+
+// 1. one segment per shared point; slot and segment reservations are
+// aggregated per wave (one atomic per wave and counter)
+__global__ __launch_bounds__(256) void k_fold_gather(const CellOut *__restrict__ outs)
+{
+    const CellOut &o = outs[blockIdx.y];
+    if (fold_overflowed(o)) return;
+    const unsigned n = o.ctrs[C_OVF];
+    const int lane = threadIdx.x & 63;
+    const unsigned long long below = (1ull << lane) - 1ull;
+    for (unsigned base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
+        const unsigned e = base + threadIdx.x;
+        bool mine = false;
+        int lidx = 0, cls = 0;
+        unsigned cnt = 0;
+        if (e < n) {
+            lidx = o.ovf[e].x;
+            mine = o.head[lidx] == (int)e;   // the newest entry of its point
+            if (mine) {
+                cnt = o.count[lidx];
+                cls = cnt <= FOLD_SMALL_MAX ? 0 : (cnt <= FOLD_MID_MAX ? 1 : 2);
+            }
+        }
+        unsigned idx = 0;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const unsigned long long m = __ballot(mine && cls == k);
+            if (m) {
+                const int leader = __ffsll((unsigned long long)m) - 1;
+                unsigned b0 = 0;
+                if (lane == leader) b0 = atomicAdd(&o.ctrs[C_FOLD0 + k], (unsigned)__popcll(m));
+                b0 = __shfl(b0, leader);
+                if (mine && cls == k) idx = b0 + (unsigned)__popcll(m & below);
+            }
+        }
+        unsigned incl = cnt;   // inclusive prefix sum of the segment sizes
+        for (int off = 1; off < 64; off <<= 1) {
+            const unsigned t = __shfl_up(incl, off);
+            if (lane >= off) incl += t;
+        }
+        const unsigned total = __shfl(incl, 63);
+        unsigned segb = 0;
+        if (total) {
+            if (lane == 0) segb = atomicAdd(&o.ctrs[C_SEG], total);
+            segb = __shfl(segb, 0);
+        }
+        if (!mine) continue;
+        const unsigned off = segb + incl - cnt;
+        if (idx >= fold_capacity(o, cls) || (unsigned long long)off + cnt > 2ull * o.ovf_cap ||
+            idx >= (1u << 28)) {
+            atomicOr(&o.ctrs[C_ERR], 8u);
+            o.head[lidx] = -(1 + 3);              // class 3: skipped by the scatter
+            continue;
+        }
+        fold_list(o, cls)[idx] = make_int4(lidx, (int)off, (int)cnt, 0);
+        o.seg[off + cnt - 1] = o.first[lidx];
+        o.head[lidx] = -(1 + (int)(idx * 4 + cls));
+    }
+}
+
+// 2. list entries into their segments
+__global__ void k_fold_scatter(const CellOut *__restrict__ outs)
+{
+    const CellOut &o = outs[blockIdx.y];
+    if (fold_overflowed(o)) return;
+    const unsigned n = o.ctrs[C_OVF];
+    for (unsigned e = blockIdx.x * blockDim.x + threadIdx.x; e < n;
+         e += gridDim.x * blockDim.x) {
+        const int4 r = o.ovf[e];
+        const int h = o.head[r.x];
+        if (h >= 0) continue;
+        const int enc = -h - 1;
+        const int cls = enc & 3;
+        if (cls == 3) continue;
+        int4 *rec = fold_list(o, cls) + (enc >> 2);
+        const unsigned k = atomicAdd(reinterpret_cast<unsigned *>(&rec->w), 1u);
+        o.seg[(unsigned)rec->y + k] = make_uint2((unsigned)r.y, (unsigned)r.z);
+    }
+}
+
+__device__ __forceinline__ void cas_pair(unsigned *kk, unsigned *mm, int j)
+{
+    const bool sw = kk[j] > kk[j + 1];
+    const unsigned k0 = sw ? kk[j + 1] : kk[j], k1 = sw ? kk[j] : kk[j + 1];
+    const unsigned m0 = sw ? mm[j + 1] : mm[j], m1 = sw ? mm[j] : mm[j + 1];
+    kk[j] = k0;
+    kk[j + 1] = k1;
+    mm[j] = m0;
+    mm[j + 1] = m1;
+}
+
+// 3a. at most 16 contributions: one lane per point, odd-even transposition
+// sort in registers
+__global__ void k_fold_small(const CellOut *__restrict__ outs, const GridP *__restrict__ gps)
+{
+    const CellOut &o = outs[blockIdx.y];
+    const float div = gps[blockIdx.y].div;
+    if (fold_overflowed(o)) return;
+    unsigned n = o.ctrs[C_FOLD0];
+    if (n > o.fold_cap[0]) n = o.fold_cap[0];
+    const double dd = (double)div;
+    for (unsigned i = blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += gridDim.x * blockDim.x) {
+        const int4 r = o.fold[0][i];
+        uint2 *sg = o.seg + (unsigned)r.y;
+        const int cnt = r.z;
+        unsigned kk[FOLD_SMALL_MAX], mm[FOLD_SMALL_MAX];
+#pragma unroll
+        for (int j = 0; j < FOLD_SMALL_MAX; ++j) {
+            kk[j] = 0xffffffffu;   // above every key
+            mm[j] = 0u;
+            if (j < cnt) {
+                const uint2 q = sg[j];
+                kk[j] = q.x;
+                mm[j] = q.y;
+            }
+        }
+#pragma unroll
+        for (int rd = 0; rd < FOLD_SMALL_MAX / 2; ++rd) {
+#pragma unroll
+            for (int j = 0; j + 1 < FOLD_SMALL_MAX; j += 2) cas_pair(kk, mm, j);
+#pragma unroll
+            for (int j = 1; j + 1 < FOLD_SMALL_MAX; j += 2) cas_pair(kk, mm, j);
+        }
+        // sorted back into the segment, then one rolled chain (no dynamic
+        // register indexing)
+#pragma unroll
+        for (int j = 0; j < FOLD_SMALL_MAX; ++j)
+            if (j < cnt) sg[j] = make_uint2(kk[j], mm[j]);
+        float rho = 0.0f;
+        for (int j = 0; j < cnt; ++j)
+            rho = (float)((double)rho + (double)__uint_as_float(sg[j].y) / dd);
+        o.density[r.x] = rho;
+    }
+}
+
+// ascending bitonic sort of P (a power of two) {key, value} pairs in LDS by
+// the threads of one workgroup
+__device__ void lds_bitonic(unsigned *sk, unsigned *sm, int P)
+{
+    const int nt = blockDim.x;
+    for (int k = 2; k <= P; k <<= 1) {
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            for (int t = threadIdx.x; t < P; t += nt) {
+                const int u = t ^ j;
+                if (u > t) {
+                    const unsigned a = sk[t], b = sk[u];
+                    if ((a > b) == ((t & k) == 0)) {
+                        sk[t] = b;
+                        sk[u] = a;
+                        const unsigned x = sm[t];
+                        sm[t] = sm[u];
+                        sm[u] = x;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+}
+
+// after sorting: the quotients mass / div in double, in place (low word in
+// sk, high word in sm), computed in parallel
+__device__ void lds_quotients(unsigned *sk, unsigned *sm, int m, double dd)
+{
+    for (int t = threadIdx.x; t < m; t += blockDim.x) {
+        const double q = (double)__uint_as_float(sm[t]) / dd;
+        const unsigned long long b = (unsigned long long)__double_as_longlong(q);
+        sk[t] = (unsigned)b;
+        sm[t] = (unsigned)(b >> 32);
+    }
+    __syncthreads();
+}
+
+__device__ __forceinline__ float lds_chain(const unsigned *sk, const unsigned *sm, int m,
+                                           float rho)
+{
+    for (int t = 0; t < m; ++t) {
+        const unsigned long long b = ((unsigned long long)sm[t] << 32) | sk[t];
+        rho = (float)((double)rho + __longlong_as_double((long long)b));
+    }
+    return rho;
+}
+
+__device__ __forceinline__ int pow2_at_least(int x)
+{
+    int p = 1;
+    while (p < x) p <<= 1;
+    return p;
+}
+
+// 3b. 17..2048 contributions: one wave per point, bitonic sort in LDS
+__global__ __launch_bounds__(64) void k_fold_mid(const CellOut *__restrict__ outs,
+                                                 const GridP *__restrict__ gps)
+{
+    __shared__ unsigned sk[FOLD_MID_MAX], sm[FOLD_MID_MAX];
+    const CellOut &o = outs[blockIdx.y];
+    const float div = gps[blockIdx.y].div;
+    if (fold_overflowed(o)) return;
+    unsigned n = o.ctrs[C_FOLD0 + 1];
+    if (n > o.fold_cap[1]) n = o.fold_cap[1];
+    const double dd = (double)div;
+    for (unsigned i = blockIdx.x; i < n; i += gridDim.x) {
+        const int4 r = o.fold[1][i];
+        const uint2 *sg = o.seg + (unsigned)r.y;
+        const int cnt = r.z;
+        const int P = pow2_at_least(cnt);
+        for (int t = threadIdx.x; t < P; t += 64) {
+            uint2 q = make_uint2(0xffffffffu, 0u);
+            if (t < cnt) q = sg[t];
+            sk[t] = q.x;
+            sm[t] = q.y;
+        }
+        __syncthreads();
+        lds_bitonic(sk, sm, P);
+        lds_quotients(sk, sm, cnt, dd);
+        if (threadIdx.x == 0) o.density[r.x] = lds_chain(sk, sm, cnt, 0.0f);
+        __syncthreads();
+    }
+}
+
+// first index i of A in the first k outputs of merge(A, B) (distinct keys)
+__device__ __forceinline__ int co_rank(const uint2 *A, int la, const uint2 *B, int lb, int k)
+{
+    int lo = k > lb ? k - lb : 0, hi = k < la ? k : la;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (A[mid].x < B[k - mid - 1].x) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// 3c. more than 2048 contributions: one 1024-lane workgroup per point;
+// chunks of 8192 sorted in LDS, then merge passes between the segment and
+// the same range of the (no longer needed) list storage
+__global__ __launch_bounds__(1024) void k_fold_large(const CellOut *__restrict__ outs,
+                                                     const GridP *__restrict__ gps)
+{
+    constexpr int CH = 8192;
+    __shared__ unsigned sk[CH], sm[CH];
+    const CellOut &o = outs[blockIdx.y];
+    const float div = gps[blockIdx.y].div;
+    if (fold_overflowed(o)) return;
+    unsigned n = o.ctrs[C_FOLD0 + 2];
+    if (n > o.fold_cap[2]) n = o.fold_cap[2];
+    const double dd = (double)div;
+    uint2 *tmp = reinterpret_cast<uint2 *>(o.ovf);   // 2 * ovf_cap pairs, like seg
+    const int nt = blockDim.x;
+    for (unsigned i = blockIdx.x; i < n; i += gridDim.x) {
+        const int4 r = o.fold[2][i];
+        const int cnt = r.z;
+        uint2 *src = o.seg + (unsigned)r.y;
+        uint2 *dst = tmp + (unsigned)r.y;
+        __syncthreads();   // the previous point is done with the LDS arrays
+        for (int c0 = 0; c0 < cnt; c0 += CH) {
+            const int m = cnt - c0 < CH ? cnt - c0 : CH;
+            const int P = pow2_at_least(m);
+            for (int t = threadIdx.x; t < P; t += nt) {
+                uint2 q = make_uint2(0xffffffffu, 0u);
+                if (t < m) q = src[c0 + t];
+                sk[t] = q.x;
+                sm[t] = q.y;
+            }
+            __syncthreads();
+            lds_bitonic(sk, sm, P);
+            for (int t = threadIdx.x; t < m; t += nt) src[c0 + t] = make_uint2(sk[t], sm[t]);
+            __syncthreads();
+        }
+        for (int w = CH; w < cnt; w <<= 1) {
+            for (int s0 = 0; s0 < cnt; s0 += 2 * w) {
+                const int la = cnt - s0 < w ? cnt - s0 : w;
+                const int rest = cnt - s0 - la;
+                const int lb = rest < w ? (rest > 0 ? rest : 0) : w;
+                const uint2 *A = src + s0, *B = src + s0 + la;
+                const long long L = (long long)la + lb;
+                const int k0 = (int)(L * threadIdx.x / nt), k1 = (int)(L * (threadIdx.x + 1) / nt);
+                int ia = co_rank(A, la, B, lb, k0), ib = k0 - ia;
+                for (int k = k0; k < k1; ++k) {
+                    const bool takeA = ib >= lb || (ia < la && A[ia].x < B[ib].x);
+                    dst[s0 + k] = takeA ? A[ia++] : B[ib++];
+                }
+            }
+            __syncthreads();
+            uint2 *x = src;
+            src = dst;
+            dst = x;
+        }
+        float rho = 0.0f;
+        for (int c0 = 0; c0 < cnt; c0 += CH) {
+            const int m = cnt - c0 < CH ? cnt - c0 : CH;
+            for (int t = threadIdx.x; t < m; t += nt) {
+                const double q = (double)__uint_as_float(src[c0 + t].y) / dd;
+                const unsigned long long b = (unsigned long long)__double_as_longlong(q);
+                sk[t] = (unsigned)b;
+                sm[t] = (unsigned)(b >> 32);
+            }
+            __syncthreads();
+            if (threadIdx.x == 0) rho = lds_chain(sk, sm, m, rho);
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) o.density[r.x] = rho;
+    }
+}
+
+static unsigned capped_blocks(unsigned long long work, unsigned per, unsigned cap)
+{
+    unsigned long long b = (work + per - 1) / per;
+    if (b > cap) b = cap;
+    if (b == 0) b = 1;
+    return (unsigned)b;
+}
+
+void launch_fold(const CellOut *outs, const GridP *gps, int nb, unsigned max_ovf_cap,
+                 const unsigned max_fold_cap[3], hipStream_t s)
+{
+    if (nb <= 0) return;
+    // grids sized by the largest block's capacities, fewer workgroups per
+    // block when there are many blocks (the loops stride)
+    const unsigned per = nb > 16 ? 256 : 4096;
+    const unsigned b = capped_blocks(max_ovf_cap, 256, per);
+    hipLaunchKernelGGL(k_fold_gather, dim3(b, nb), dim3(256), 0, s, outs);
+    hipLaunchKernelGGL(k_fold_scatter, dim3(b, nb), dim3(256), 0, s, outs);
+    hipLaunchKernelGGL(k_fold_small, dim3(capped_blocks(max_fold_cap[0], 256, per), nb),
+                       dim3(256), 0, s, outs, gps);
+    hipLaunchKernelGGL(k_fold_mid, dim3(capped_blocks(max_fold_cap[1], 1, per), nb), dim3(64), 0,
+                       s, outs, gps);
+    hipLaunchKernelGGL(k_fold_large, dim3(capped_blocks(max_fold_cap[2], 1, 64), nb),
+                       dim3(1024), 0, s, outs, gps);
+}
+
+}  // namespace t2

@@ -26,13 +26,33 @@ struct GridP {
 // counters in device memory (ctrs[])
 enum {
     C_OVF = 0,        // entries appended to the shared-point list
-    C_OFFBLOCK = 1,   // points outside the block bounds (dropped)
+    C_OFFBLOCK = 1,   // points no block grid holds (dropped)
     C_SLOW = 2,       // cells queued for the large-capacity pass
-    C_ERR = 3,        // bit 0: deferred-cell list full; bit 2: cell beyond all capacities
-    C_OFFGRID = 4,    // in-bounds points whose index left the block grid
+    C_ERR = 3,        // bit 0: deferred-cell list full; bit 2: cell beyond all capacities;
+                      // bit 3: a shared point beyond the fold's capacities
+    C_OFFGRID = 4,    // (unused)
     C_FALLBACK = 5,   // FP64 re-checks of the exact assignment
-    C_NCTR = 8
+    C_ROUTED = 6,     // points deposited into another block's grid
+    C_SEG = 7,        // contributions gathered into per-point segments for the fold
+    C_FOLD0 = 8,      // shared points per fold class (C_FOLD0 + class, 3 classes)
+    C_NCTR = 12
 };
+
+// fold classes by contributions per shared point
+enum { FOLD_SMALL_MAX = 16, FOLD_MID_MAX = 2048 };
+
+// Counters and statistics of one block; the records of all blocks of a
+// context sit in one device array (cleared by one fill, copied to pinned host
+// memory by one transfer).
+struct BlockStats {
+    unsigned ctrs[C_NCTR];
+    unsigned pad0[16 - C_NCTR];
+    unsigned long long cstat[6];   // cells per status 0..4, points of deposited cells
+    double gsum;                   // sum of the block's density grid
+    unsigned gmax;                 // bits of its maximum (non-negative float)
+    unsigned pad1;
+};
+static_assert(sizeof(BlockStats) % 16 == 0, "BlockStats is cleared with 16-byte stores");
 
 // Per-block device outputs of the density kernels.
 struct CellOut {
@@ -48,6 +68,12 @@ struct CellOut {
     unsigned *ctrs;    // [C_NCTR]
     int *slow;         // cells for the large-capacity pass
     unsigned slow_cap;
+    uint2 *seg;        // [2 * ovf_cap] contributions {key, mass bits}, one segment per point
+    int4 *fold[3];     // shared points per fold class: {lidx, segment start, count, cursor}
+    unsigned fold_cap[3];
+    BlockStats *st;    // this block's record (st->ctrs == ctrs)
+    unsigned long long ngrid;
+    int ncells;
 };
 
 // uniform bucket hash of the particles: bucket (i,j,k) holds the particles
@@ -59,6 +85,23 @@ struct HashP {
     int dims[3];
 };
 
+// Per-block grids and outputs of all blocks of a run, in staging order (the
+// rank of a block = its position; blocks are staged in ascending gid), for
+// deposits that leave their block.
+struct Router {
+    int nblocks;
+    const GridP *gps;
+    const CellOut *outs;
+};
+
+// one chunk of cells of one block for the batched launch
+struct Chunk {
+    int block;
+    int start;   // position in the block's cell order
+    int count;
+    int pad;
+};
+
 struct BlockIn {
     const float4 *pts;   // xyz + pad
     const int4 *tets;    // 2 int4 per tet: verts, neighbours
@@ -68,6 +111,30 @@ struct BlockIn {
     int ncells;          // original particles
     int ntets;
 };
+
+// find or insert key (>= 0) in an open-addressing table of HS entries in
+// LDS (-1 = empty); returns the entry, or -1 when HS probes found no room.
+// inserted: this lane's CAS created the entry (one lane per new key).
+template <int HS>
+__device__ __forceinline__ int hash_find(int *hk, int key, bool &inserted)
+{
+    unsigned h = ((unsigned)key * 0x9E3779B1u) >> 16;
+    inserted = false;
+    for (int n = 0; n < HS; ++n) {
+        h &= HS - 1;
+        const int old = atomicCAS(&hk[h], -1, key);
+        if (old == -1) inserted = true;
+        if (old == -1 || old == key) return (int)h;
+        ++h;
+    }
+    return -1;
+}
+template <int HS>
+__device__ __forceinline__ int hash_find(int *hk, int key)
+{
+    bool ins;
+    return hash_find<HS>(hk, key, ins);
+}
 
 // Workgroup -> work item with XCD affinity: the dispatcher hands consecutive
 // workgroups to the 8 XCDs in turn, so item k = (b % 8) * chunk + b / 8 gives
@@ -94,15 +161,24 @@ void launch_exact_status(const BlockIn &in, const GridP &gp, int *status, int pr
 void launch_exact_tail(const BlockIn &in, const GridP &gp, const int *owner,
                        const int *status_ex, const CellOut &o, int prec, hipStream_t s);
 void launch_fill_u32(unsigned *p, size_t n, unsigned v, hipStream_t s);
+// zero every block's density and count grids; circumcenters of every block
+void launch_block_prep(const BlockIn *ins, const CellOut *outs, int nb, int max_tets,
+                       unsigned long long max_grid, bool circumcenters, hipStream_t s);
 void launch_cells_ref(const BlockIn &in, const GridP &gp, const CellOut &o,
                       int want_density, hipStream_t s);
 void launch_cells_ref_slow(const BlockIn &in, const GridP &gp, const CellOut &o,
                            int want_density, hipStream_t s);
-void launch_cic(const BlockIn &in, const GridP &gp, const CellOut &o, hipStream_t s);
-void launch_finalize(const CellOut &o, const GridP &gp, hipStream_t s);
-void launch_cell_stats(const int *status, const int *npts, int ncells,
-                       unsigned long long *out6, hipStream_t s);
-void launch_grid_stats(const float *density, size_t n, double *sum, unsigned *maxbits,
-                       hipStream_t s);
+void launch_cic(const BlockIn &in, const GridP &gp, const CellOut &o, const Router *rt_dev,
+                int self, hipStream_t s);
+void launch_cells_ref_batched(const BlockIn *ins, const GridP *gps, const CellOut *outs,
+                              const Router *rt, const Chunk *chunks, int nchunks,
+                              unsigned *queue, int waves, int want_density, hipStream_t s);
+// ordered fold of shared grid points and per-block statistics for nb blocks
+// at once (outs/gps: device arrays; the capacities size the grids)
+void launch_fold(const CellOut *outs, const GridP *gps, int nb, unsigned max_ovf_cap,
+                 const unsigned max_fold_cap[3], hipStream_t s);
+// part: nb * 256 doubles of scratch for the fixed-order grid sums
+void launch_stats(const CellOut *outs, double *part, int nb, int max_cells,
+                  unsigned long long max_grid, hipStream_t s);
 
 }  // namespace t2

@@ -113,9 +113,25 @@ def _block_grids(blocks, num_given, given_mins, given_maxs, glo_num_idx, eps):
 def dense(alg, num_given_bounds, given_mins, given_maxs, project, proj_plane, mass, eps,
           glo_num_idx, blocks, assignment=REFERENCE_SCAN, precision=MIXED, ctx=None,
           want_cells=False):
-    """dense() over host blocks: density of every block on its block grid."""
+    """dense() over host blocks: density of every block on its block grid.
+    Results come back in the order of `blocks`; the library sees them in
+    ascending gid order (the order received points are added in)."""
     ctx = ctx or context()
     blocks = list(blocks)
+    perm = sorted(range(len(blocks)), key=lambda i: getattr(blocks[i], "gid", 0))
+    if perm != list(range(len(blocks))):
+        res = dense(alg, num_given_bounds, given_mins, given_maxs, project, proj_plane, mass,
+                    eps, glo_num_idx, [blocks[i] for i in perm], assignment, precision, ctx,
+                    want_cells)
+        inv = [0] * len(perm)
+        for k, i in enumerate(perm):
+            inv[i] = k
+        res.density = [res.density[inv[i]] for i in range(len(perm))]
+        res.block_num_idx = [res.block_num_idx[inv[i]] for i in range(len(perm))]
+        res.block_min_idx = [res.block_min_idx[inv[i]] for i in range(len(perm))]
+        if want_cells:
+            res.cells = [res.cells[inv[i]] for i in range(len(perm))]
+        return res
     nb = len(blocks)
     cs = (N.Block * nb)()
     keep = []
@@ -168,11 +184,12 @@ def write_grid(path, density, glo_num_idx, project=False):
 
 
 class StagedRun:
-    """Blocks resident in HBM; each run() replays the whole path on the device."""
+    """Blocks resident in HBM; each run() replays the whole path on the device.
+    Blocks are kept in ascending gid order (fetch/cells index that order)."""
 
     def __init__(self, blocks, ctx=None):
         self.ctx = ctx or N.Context(0)
-        self.blocks = list(blocks)
+        self.blocks = sorted(blocks, key=lambda b: getattr(b, "gid", 0))
         nb = len(self.blocks)
         cs = (N.Block * nb)()
         keep = []
@@ -191,6 +208,17 @@ class StagedRun:
                                          float(np.float32(mass)), float(np.float32(eps)),
                                          N.i3(glo_num_idx), int(assignment), int(precision),
                                          int(bool(want_volumes))), "run")
+        grids = _block_grids(self.blocks, num_given_bounds, given_mins, given_maxs,
+                             glo_num_idx, np.float32(eps))[5]
+        self.ngrid = [int(g[2][0]) * int(g[2][1]) * (1 if project else int(g[2][2]))
+                      for g in grids]
+
+    def _grid_size(self, b, ngrid):
+        """Grid points of block b in the last run (the library copies that many)."""
+        n = self.ngrid[b]
+        if ngrid is not None and int(ngrid) != n:
+            raise ValueError(f"block {b} has {n} grid points, not {ngrid}")
+        return n
 
     def sync(self) -> dict:
         st = N.Stats()
@@ -200,7 +228,7 @@ class StagedRun:
     def fetch(self, b, ngrid=None, volumes=False, circumcenters=False):
         hb = self.blocks[b]
         out = {}
-        d = np.zeros(ngrid, np.float32) if ngrid else None
+        d = np.zeros(self._grid_size(b, ngrid), np.float32) if ngrid is not False else None
         v = np.zeros(hb.num_orig, np.float32) if volumes else None
         c = np.zeros((hb.num_tets, 3), np.float32) if circumcenters else None
         self.ctx.check(N.lib().tess2_fetch(self.ctx.h, int(b),
@@ -210,8 +238,8 @@ class StagedRun:
         out["density"], out["volumes"], out["circumcenters"] = d, v, c
         return out
 
-    def owner(self, b, ngrid):
-        o = np.zeros(ngrid, np.int32)
+    def owner(self, b, ngrid=None):
+        o = np.zeros(self._grid_size(b, ngrid), np.int32)
         self.ctx.check(N.lib().tess2_fetch_owner(self.ctx.h, int(b), o.ctypes.data), "owner")
         return o
 
@@ -229,3 +257,25 @@ class StagedRun:
         self.ctx.check(N.lib().tess2_kernel_time(self.ctx.h, C.byref(ms), C.byref(n)),
                        "kernel_time")
         return ms.value, n.value
+
+    OPTIONS = {"graph": N.OPT_GRAPH, "waves_per_cu": N.OPT_WAVES_PER_CU,
+               "chunk_small": N.OPT_CHUNK_SMALL, "chunk_medium": N.OPT_CHUNK_MEDIUM,
+               "chunk_large": N.OPT_CHUNK_LARGE}
+
+    def set_option(self, name, value):
+        """Launch option (graph replay, persistent waves, chunk sizes)."""
+        self.ctx.check(N.lib().tess2_set_option(self.ctx.h, self.OPTIONS[name], int(value)),
+                       "set_option " + name)
+
+    def get_option(self, name):
+        v = C.c_int()
+        self.ctx.check(N.lib().tess2_get_option(self.ctx.h, self.OPTIONS[name], C.byref(v)),
+                       "get_option " + name)
+        return v.value
+
+    def autotune(self, reps=3):
+        """Pick the fastest work split for the last run's arguments; returns
+        the best median run time (ms) and the chosen options."""
+        ms = C.c_float()
+        self.ctx.check(N.lib().tess2_autotune(self.ctx.h, int(reps), C.byref(ms)), "autotune")
+        return ms.value, {k: self.get_option(k) for k in self.OPTIONS}

@@ -281,3 +281,45 @@ def periodic_block(p: np.ndarray, ghost_width: float, workers: int = 1,
 def default_workers() -> int:
     n = os.cpu_count() or 1
     return max(1, min(16, n))
+
+
+def _block_job(args):
+    allp, norig, lo, hi, gid = args
+    t0 = time.time()
+    tets = qhull_tets(allp)
+    return tets, time.time() - t0
+
+
+def decomposed_blocks(points: np.ndarray, block_id: np.ndarray, bounds, ghost_width: float,
+                      workers: int = 1, log=None) -> list:
+    """DIY-style blocks: block b holds the particles with block_id == b (its
+    original particles, in input order) followed by the particles of the
+    other blocks within ghost_width of its bounds (its ghosts), and its own
+    Qhull tessellation (src/tess.cpp:351-457 computes one per block after the
+    ghost exchange).  Blocks are tessellated in parallel processes.
+    bounds: list of (lo, hi) per block; gid = block index."""
+    points = np.ascontiguousarray(points, dtype=np.float32)
+    w = np.float32(ghost_width)
+    jobs = []
+    for b, (lo, hi) in enumerate(bounds):
+        lo = np.asarray(lo, np.float32)
+        hi = np.asarray(hi, np.float32)
+        own = points[block_id == b]
+        near = (block_id != b) & np.all(points >= lo - w, axis=1) & np.all(points <= hi + w, axis=1)
+        allp = np.concatenate([own, points[near]]).astype(np.float32)
+        jobs.append((allp, own.shape[0], lo, hi, b))
+    if workers > 1 and len(jobs) > 1:
+        from concurrent.futures import ProcessPoolExecutor
+        with ProcessPoolExecutor(max_workers=workers) as ex:
+            results = list(ex.map(_block_job, jobs))
+    else:
+        results = [_block_job(j) for j in jobs]
+    blocks = []
+    for (allp, norig, lo, hi, b), (tets, dt) in zip(jobs, results):
+        v2t = vert_to_tet(tets, allp.shape[0])
+        blocks.append(HostBlock(particles=allp, tets=tets, vert_to_tet=v2t, num_orig=norig,
+                                bounds_min=lo, bounds_max=hi, gid=b, info=dict(qhull_s=dt)))
+        if log:
+            log(f"  block {b}: {norig} + {allp.shape[0] - norig} ghosts, {tets.shape[0]} tets, "
+                f"{dt:.1f}s")
+    return blocks

@@ -108,6 +108,9 @@ def plummer_blocks(nblocks_side: int = 4, nmin: int = 5000, nmax: int = 200000,
 CONFIGS = {
     0: dict(name="uniform32k_64", n_side=32, grid=64, kind="uniform", ghost=4.0),
     1: dict(name="grf2M_256", n_side=128, grid=256, kind="grf", rms=0.3, ghost=4.0),
+    # 4^3 blocks, each on a 64^3 block grid (global 256^3); ghost 0.1 block width
+    2: dict(name="plummer64_256", kind="plummer", nblocks_side=4, nmin=5000, nmax=200000,
+            grid=256, ghost_blocks=0.1),
     3: dict(name="grf16M_512", n_side=256, grid=512, kind="grf", rms=0.5, ghost=4.0),
     4: dict(name="lattice2M_512", n_side=128, grid=512, kind="lattice", jitter=1e-6,
             ghost=3.0),
@@ -123,4 +126,6 @@ def config_particles(idx: int, n_side: int | None = None) -> np.ndarray:
         return grf_lattice(ns, c["rms"], seed=idx)
     if c["kind"] == "lattice":
         return jittered_lattice(ns, c["jitter"], seed=idx)
+    if c["kind"] == "plummer":
+        raise ValueError("the plummer workload is a block decomposition: use plummer_blocks")
     raise ValueError(idx)

@@ -246,3 +246,29 @@ def test_exact_dense_oracle_invariants(oracle):
         assert (own == c).sum() == n[c]
     div = float(step[0] * step[1] * step[2])          # float product, as the reference
     assert abs(r["density"].sum() * div - 3.0 * dep.sum()) < 1e-9 * 3.0 * dep.sum() + 1e-6
+
+
+def test_multiblock_model_conserves_mass(oracle):
+    """Oracle multi-block model: every deposited cell's mass lands in some
+    block grid (own or received) unless its point is held by no block."""
+    rng = np.random.default_rng(61)
+    w = 0.5
+    pts, bid, bounds = [], [], []
+    for b in range(8):
+        lo = np.array([b % 2, (b // 2) % 2, b // 4]) * w
+        pts.append((lo + rng.random((700, 3)) * w).astype(np.float32))
+        bid.append(np.full(700, b))
+        bounds.append((lo.astype(np.float32), (lo + w).astype(np.float32)))
+    blocks = ingest.decomposed_blocks(np.concatenate(pts), np.concatenate(bid), bounds, 0.12)
+    obs = [oracle.Block.from_host(b) for b in blocks]
+    r = oracle.dense_multi(obs, (33, 33, 33), mass=2.0)
+    div = float(r["grid_step_size"][0] * r["grid_step_size"][1] * r["grid_step_size"][2])
+    tot = sum(d.astype(np.float64).sum() for d in r["density"]) * div
+    ncell = sum(int(((s == 0) | (s == 1)).sum()) for s in r["status"])
+    assert r["dropped"] == 0
+    assert abs(tot - 2.0 * ncell) < 1e-4 * 2.0 * ncell
+    # the block grids partition the global grid
+    cover = np.zeros((33, 33, 33), int)
+    for mn, mx, nm in r["grids"]:
+        cover[mn[2]:mx[2] + 1, mn[1]:mx[1] + 1, mn[0]:mx[0] + 1] += 1
+    assert (cover == 1).all()

@@ -405,69 +405,9 @@ __device__ void cell_ref_body(const int v, const BlockIn &in, const GridP &gp,
             return !(__ballot(pos) != 0ull && __ballot(neg) != 0ull);
         };
 
-        // Packed PtInCell: with F <= 32 faces the wave tests G = 64 / gs points
-        // at once, lane group g (gs lanes, one face each) holding point g.  The
-        // per-face arithmetic is that of inside(), so the verdicts are the
-        // same; the scan reads them from a per-row cache (bit x = verdict at
-        // x) filled a window of G points at a time.  PtInCell is pure, so
-        // testing points the scan never asks for changes nothing.
-        int gs = 64;
-        if (NF == 1) gs = F <= 8 ? 8 : (F <= 16 ? 16 : (F <= 32 ? 32 : 64));
-        const int G = 64 / gs;
-        const int gshift = gs == 8 ? 3 : (gs == 16 ? 4 : (gs == 32 ? 5 : 6));
-        const int fl = lane & (gs - 1), grp = lane >> gshift;
-        const float rnx = __shfl(fnx[0], fl), rny = __shfl(fny[0], fl), rnz = __shfl(fnz[0], fl);
-        const float rfx = __shfl(ffx[0], fl), rfy = __shfl(ffy[0], fl), rfz = __shfl(ffz[0], fl);
-        const unsigned long long gmask = gs == 64 ? ~0ull : ((1ull << gs) - 1ull);
-        // verdicts (bit g) of the points (xs + dx*g, ys + dy*g, zs), g < G
-        auto packed = [&](int xs, int ys, int zs, int dx, int dy) -> unsigned {
-            const float qx = p0 + (float)(xs + dx * grp) * gp.step[0];
-            const float qy = p1 + (float)(ys + dy * grp) * gp.step[1];
-            const float qz = p2 + (float)zs * gp.step[2];
-            bool pos = false, neg = false;
-            if (fl < F) {
-                const float d = rnx * (qx - rfx) + rny * (qy - rfy) + rnz * (qz - rfz);
-                if (fabsf(d) > eps) {
-                    if (d >= 0.0f) pos = true;
-                    else neg = true;
-                }
-            }
-            const unsigned long long P = __ballot(pos), Q = __ballot(neg);
-            unsigned res = 0;
-            for (int g = 0; g < G; ++g)
-                if (!(((P >> (g * gs)) & gmask) && ((Q >> (g * gs)) & gmask))) res |= 1u << g;
-            return res;
-        };
-        const bool packed_ok = NF == 1 && n0 <= 64;
-        int ry = -1, rz = -1;
-        unsigned long long rvalid = 0, rin = 0;
-        // one point: every group tests it (NF == 1), or all faces of the
-        // larger-capacity kernel at once
 #define PROBE(xi, yi, zi)                                                                     \
-    (NF == 1 ? (packed(xi, yi, zi, 0, 0) & 1u) != 0u                                           \
-             : inside(p0 + (float)(xi) * gp.step[0], p1 + (float)(yi) * gp.step[1],           \
-                      p2 + (float)(zi) * gp.step[2]))
-#define ROW_PROBE(xi, yi, zi)                                                                 \
-    (packed_ok ? row_bit(xi, yi, zi) : PROBE(xi, yi, zi))
-        auto row_bit = [&](int xi, int yi, int zi) -> bool {
-            if (yi != ry || zi != rz) {
-                ry = yi;
-                rz = zi;
-                rvalid = 0;
-                rin = 0;
-            }
-            if (!((rvalid >> xi) & 1ull)) {
-                int xs = xi - (G - 1) / 2;
-                if (xs > n0 - G) xs = n0 - G;
-                if (xs < 0) xs = 0;
-                const unsigned res = packed(xs, yi, zi, 1, 0);
-                const int cnt = n0 - xs < G ? n0 - xs : G;
-                const unsigned long long wm = (cnt >= 64 ? ~0ull : ((1ull << cnt) - 1ull)) << xs;
-                rvalid |= wm;
-                rin |= ((unsigned long long)res << xs) & wm;
-            }
-            return (rin >> xi) & 1ull;
-        };
+    inside(p0 + (float)(xi) * gp.step[0], p1 + (float)(yi) * gp.step[1],                       \
+           p2 + (float)(zi) * gp.step[2])
 
         // ---- 5. scanline search (src/dense.cpp:1486-1676) --------------------
         // Pass 0 counts the points and keeps the runs (one per z-y line) in
@@ -486,12 +426,10 @@ __device__ void cell_ref_body(const int v, const BlockIn &in, const GridP &gp,
             for (int zi = 0; zi < n2; zi++) {
                 bool border_found = false, z_step_done = false;
                 for (int yi = y_start; yi < n1; yi++) {
-                    // the verdict at x0 is reused by both walks (PtInCell is pure)
-                    const int x0 = x_left;
-                    const bool x_in = ROW_PROBE(x0, yi, zi);
+                    const bool x_in = PROBE(x_left, yi, zi);
                     int min_xi = n0 - 1, max_xi = 0;
                     for (int xi = x_left; xi >= 0 && xi < n0;) {
-                        if (xi == x0 ? x_in : ROW_PROBE(xi, yi, zi)) {
+                        if (PROBE(xi, yi, zi)) {
                             if (xi < min_xi) min_xi = xi;
                             if (xi > max_xi) max_xi = xi;
                             if (x_in) xi--;
@@ -508,7 +446,7 @@ __device__ void cell_ref_body(const int v, const BlockIn &in, const GridP &gp,
                         }
                     }
                     for (int xi = x_right; xi >= 0 && xi < n0;) {
-                        if (xi == x0 ? x_in : ROW_PROBE(xi, yi, zi)) {
+                        if (PROBE(xi, yi, zi)) {
                             if (xi < min_xi) min_xi = xi;
                             if (xi > max_xi) max_xi = xi;
                             if (x_in) xi++;
@@ -547,24 +485,9 @@ __device__ void cell_ref_body(const int v, const BlockIn &in, const GridP &gp,
                         z_step_done = true;
                     }
                     if ((yi == n1 - 1 || z_step_done) && zi + 1 < n2) {
-                        int yj = first_y;
-                        if (NF == 1 && G > 1) {
-                            // G points down the column at once
-                            bool stop = false;
-                            while (yj > 0 && !stop) {
-                                const unsigned res = packed(first_x, yj, zi + 1, 0, -1);
-                                for (int g = 0; g < G && yj > 0; ++g) {
-                                    if (!((res >> g) & 1u)) {
-                                        stop = true;
-                                        break;
-                                    }
-                                    --yj;
-                                }
-                            }
-                        } else {
-                            for (; yj > 0; yj--) {
-                                if (!PROBE(first_x, yj, zi + 1)) break;
-                            }
+                        int yj;
+                        for (yj = first_y; yj > 0; yj--) {
+                            if (!PROBE(first_x, yj, zi + 1)) break;
                         }
                         y_start = yj;
                     }
@@ -573,7 +496,6 @@ __device__ void cell_ref_body(const int v, const BlockIn &in, const GridP &gp,
             }
         }
 #undef PROBE
-#undef ROW_PROBE
         __syncthreads();
 
         // ---- 6. deposit -------------------------------------------------------
@@ -668,12 +590,12 @@ __global__ __launch_bounds__(64) void k_cells_ref_slow(BlockIn in, GridP gp, Cel
     }
 }
 
-// All blocks of a run in one persistent launch (at least 6 waves per SIMD:
-// the per-block descriptors read from memory otherwise push the register
-// count to 4 waves; measured 30 vs 34.5 ms on 64 Plummer blocks): every wave takes chunks of
+// All blocks of a run in one persistent launch: every wave takes chunks of
 // cells (one block each, consecutive in the block's Morton order) from a
 // global queue until the queue is empty.  Chunks are listed largest block
-// first, with chunk sizes by block size class (tess2 autotuner).
+// first, with chunk sizes by block size class (tess2_autotune).  At least 6
+// waves per SIMD: the per-block descriptors read from memory otherwise push
+// the register count down to 4 waves (30 vs 34.5 ms on 64 Plummer blocks).
 template <int NS, int NF, int RMAX>
 __global__ __launch_bounds__(64, 6) void k_cells_ref_batched(const BlockIn *__restrict__ ins,
                                                           const GridP *__restrict__ gps,

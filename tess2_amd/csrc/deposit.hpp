@@ -21,33 +21,12 @@ __device__ __forceinline__ void push_point(long long lidx, float m, unsigned key
     if (c == 0u) {
         o.first[lidx] = make_uint2(key, __float_as_uint(m));
         o.density[lidx] = (float)((double)m / (double)div);
-    }
-    // list entries: one atomic per wave and block counter (the active lanes
-    // of a wave may deposit into several blocks)
-    const int lane = threadIdx.x & 63;
-    unsigned *ctr = &o.ctrs[C_OVF];
-    const unsigned long long cp = reinterpret_cast<unsigned long long>(ctr);
-    bool pend = c != 0u;
-    unsigned e = 0;
-    for (;;) {
-        const unsigned long long act = __ballot(pend);
-        if (!act) break;
-        const int leader = __ffsll((unsigned long long)act) - 1;
-        const unsigned lo = __builtin_amdgcn_readlane((unsigned)cp, leader);
-        const unsigned hi = __builtin_amdgcn_readlane((unsigned)(cp >> 32), leader);
-        const bool same = pend && (unsigned)cp == lo && (unsigned)(cp >> 32) == hi;
-        const unsigned long long sm = __ballot(same);
-        unsigned b0 = 0;
-        if (lane == leader) b0 = atomicAdd(ctr, (unsigned)__popcll(sm));
-        b0 = __shfl(b0, leader);
-        if (same) {
-            e = b0 + (unsigned)__popcll(sm & ((1ull << lane) - 1ull));
-            pend = false;
+    } else {
+        const unsigned e = atomicAdd(&o.ctrs[C_OVF], 1u);
+        if (e < o.ovf_cap) {
+            const int prev = atomicExch(&o.head[lidx], (int)e);
+            o.ovf[e] = make_int4((int)lidx, (int)key, __float_as_int(m), prev);
         }
-    }
-    if (c != 0u && e < o.ovf_cap) {
-        const int prev = atomicExch(&o.head[lidx], (int)e);
-        o.ovf[e] = make_int4((int)lidx, (int)key, __float_as_int(m), prev);
     }
 }
 

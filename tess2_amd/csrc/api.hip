@@ -48,6 +48,7 @@ struct DevBlock {
     // exact-Voronoi assignment
     int *owner = nullptr;                    // [grid] nearest site
     int *status_ex = nullptr;                // [ncells] deposit decision
+    unsigned *exflags = nullptr;             // [3 * ncells bytes] its per-vertex flags
     HashP hash{};
     unsigned *hstart = nullptr, *hfill = nullptr, *hsums = nullptr;
     int *hbk = nullptr;
@@ -90,6 +91,7 @@ struct tess2_ctx {
     Chunk *d_chunks = nullptr;
     int nchunks = 0;
     unsigned *d_queue = nullptr;
+    XcdQueues xq{};
     std::vector<BlockIn> h_ins;
     std::vector<GridP> h_gps;
     std::vector<CellOut> h_outs;
@@ -181,7 +183,7 @@ static void free_block(DevBlock &b)
 {
     void *ps[] = {b.order,  b.pts,    b.tets,   b.v2t,   b.cc,   b.vol,  b.status,    b.npts,
                   b.density, b.count, b.first, b.head, b.ovf,  b.slow,
-                  b.owner, b.status_ex, b.hstart, b.hfill,
+                  b.owner, b.status_ex, b.exflags, b.hstart, b.hfill,
                   b.hsums,  b.hbk,    b.hsorted, b.seg, b.fold[0], b.fold[1], b.fold[2]};
     for (void *p : ps)
         if (p) (void)hipFree(p);
@@ -386,13 +388,12 @@ static int validate(tess2_ctx *c, const tess2_block_t &b)
 // decreasing size so that the big ones start first.
 static int build_chunks(tess2_ctx *c)
 {
-    std::vector<int> ord(c->blocks.size());
-    for (size_t i = 0; i < ord.size(); ++i) ord[i] = (int)i;
-    std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) {
-        return c->blocks[a].norig > c->blocks[b].norig;
-    });
+    // blocks in staging (gid) order, each block's cells in Morton order, so
+    // that the 8 XCD ranges below are compact in space
     std::vector<Chunk> ch;
-    for (int b : ord) {
+    long long total = 0;
+    for (const DevBlock &d : c->blocks) total += d.norig;
+    for (int b = 0; b < (int)c->blocks.size(); ++b) {
         const int n = c->blocks[b].norig;
         const int cs = n < 20000 ? c->chunk_small : (n < 100000 ? c->chunk_medium : c->chunk_large);
         for (int s0 = 0; s0 < n; s0 += cs) {
@@ -405,6 +406,15 @@ static int build_chunks(tess2_ctx *c)
         }
     }
     c->nchunks = (int)ch.size();
+    // split into 8 ranges of about equal cell counts
+    long long acc = 0;
+    int x = 1;
+    c->xq.start[0] = 0;
+    for (int i = 0; i < (int)ch.size() && x < 8; ++i) {
+        acc += ch[i].count;
+        while (x < 8 && acc * 8 >= total * x) c->xq.start[x++] = i + 1;
+    }
+    while (x <= 8) c->xq.start[x++] = (int)ch.size();
     int r = dalloc(c, &c->d_chunks, std::max<size_t>(ch.size(), 1));
     if (r) return r;
     if (!ch.empty())
@@ -511,14 +521,15 @@ extern "C" int tess2_stage(tess2_ctx *c, const tess2_block_t *blocks, int nblock
         if ((r = dalloc(c, &d.hstart, nbk + 1)) || (r = dalloc(c, &d.hfill, nbk + 1)) ||
             (r = dalloc(c, &d.hsums, (nbk + 1 + 1023) / 1024)) ||
             (r = dalloc(c, &d.hbk, (size_t)d.np)) || (r = dalloc(c, &d.hsorted, (size_t)d.np)) ||
-            (r = dalloc(c, &d.status_ex, (size_t)d.norig)))
+            (r = dalloc(c, &d.status_ex, (size_t)d.norig)) ||
+            (r = dalloc(c, &d.exflags, ((size_t)d.norig * 3 + 3) / 4)))
             return r;
     }
     if (nblocks > 0) {
         int r = 0;
         if ((r = dalloc(c, &c->d_ins, (size_t)nblocks)) || (r = dalloc(c, &c->d_gps, (size_t)nblocks)) ||
             (r = dalloc(c, &c->d_outs, (size_t)nblocks)) || (r = dalloc(c, &c->d_router, 1)) ||
-            (r = dalloc(c, &c->d_queue, 4)) || (r = dalloc(c, &c->d_stats, (size_t)nblocks)) ||
+            (r = dalloc(c, &c->d_queue, 16)) || (r = dalloc(c, &c->d_stats, (size_t)nblocks)) ||
             (r = dalloc(c, &c->d_gpart, (size_t)nblocks * 256)))
             return r;
         // counters and statistics of all blocks: one device array, one pinned
@@ -725,7 +736,8 @@ static int enqueue_kernels(tess2_ctx *c, const RunArgs &a)
             launch_cells_ref_slow(in, gp, ov, 0, s);
         }
         launch_hash_build(d.pts, d.np, d.hash, d.hstart, d.hfill, d.hsums, d.hbk, d.hsorted, s);
-        launch_exact_status(in, gp, d.status_ex, a.precision, d.ctrs, d.slow, d.slow_cap, s);
+        launch_exact_status(in, gp, d.status_ex, a.precision, d.ctrs,
+                            reinterpret_cast<unsigned char *>(d.exflags), s);
         if (ev) HIPCHK(c, hipEventRecord(c->ev_k0, s));
         launch_owner(gp, d.hash, d.hstart, d.hsorted, d.owner, a.precision, d.ctrs, d.status_ex,
                      d.npts, d.norig, s);
@@ -744,8 +756,8 @@ static int enqueue_kernels(tess2_ctx *c, const RunArgs &a)
         launch_cells_ref_slow(c->h_ins[0], c->h_gps[0], c->h_outs[0], 1, s);
     } else {
         if (ev) HIPCHK(c, hipEventRecord(c->ev_k0, s));
-        launch_cells_ref_batched(c->d_ins, c->d_gps, c->d_outs, c->d_router, c->d_chunks,
-                                 c->nchunks, c->d_queue, c->ncu * c->waves_per_cu, 1, s);
+        launch_cells_ref_batched(c->d_ins, c->d_gps, c->d_outs, c->d_router, c->d_chunks, c->xq,
+                                 c->d_queue, c->ncu * c->waves_per_cu, 1, s);
         if (ev) HIPCHK(c, hipEventRecord(c->ev_k1, s));
     }
     // phase C: ordered fold of shared points, statistics, one copy of the

@@ -591,9 +591,9 @@ __global__ __launch_bounds__(64) void k_cells_ref_slow(BlockIn in, GridP gp, Cel
 }
 
 // All blocks of a run in one persistent launch: every wave takes chunks of
-// cells (one block each, consecutive in the block's Morton order) from a
-// global queue until the queue is empty.  Chunks are listed largest block
-// first, with chunk sizes by block size class (tess2_autotune).  At least 6
+// cells (one block each, consecutive in the block's Morton order) from the
+// queue of its XCD's range of the chunk list, then from the other ranges,
+// until all are empty.  Chunk sizes by block size class (tess2_autotune).  At least 6
 // waves per SIMD: the per-block descriptors read from memory otherwise push
 // the register count down to 4 waves (30 vs 34.5 ms on 64 Plummer blocks).
 template <int NS, int NF, int RMAX>
@@ -602,7 +602,7 @@ __global__ __launch_bounds__(64, 6) void k_cells_ref_batched(const BlockIn *__re
                                                           const CellOut *__restrict__ outs,
                                                           const Router *__restrict__ rt,
                                                           const Chunk *__restrict__ chunks,
-                                                          int nchunks, unsigned *queue,
+                                                          XcdQueues xq, unsigned *queue,
                                                           int want_density)
 {
     __shared__ int4 s_v[64 * NS];
@@ -611,11 +611,16 @@ __global__ __launch_bounds__(64, 6) void k_cells_ref_batched(const BlockIn *__re
     __shared__ int4 s_u[(64 * NS > RMAX) ? 64 * NS : RMAX];
     __shared__ int s_h[hash_ints<NS>()];
     const int lane = lane_id();
-    for (;;) {
+    const int x0 = (int)(blockIdx.x & 7);
+    for (int q = 0; q < 8;) {
+        const int x = (x0 + q) & 7;
         int c = 0;
-        if (lane == 0) c = (int)atomicAdd(queue, 1u);
-        c = __builtin_amdgcn_readfirstlane(c);
-        if (c >= nchunks) break;
+        if (lane == 0) c = (int)atomicAdd(&queue[x], 1u);
+        c = xq.start[x] + __builtin_amdgcn_readfirstlane(c);
+        if (c >= xq.start[x + 1]) {   // range x is done: help the next one
+            ++q;
+            continue;
+        }
         const Chunk ch = chunks[c];
         // read-only descriptors (scalar loads; rt->gps/outs alias them for routing)
         const BlockIn in = ins[ch.block];
@@ -655,13 +660,13 @@ __global__ __launch_bounds__(64) void k_cells_ref_slow_batched(const BlockIn *__
 }
 
 void launch_cells_ref_batched(const BlockIn *ins, const GridP *gps, const CellOut *outs,
-                              const Router *rt, const Chunk *chunks, int nchunks,
+                              const Router *rt, const Chunk *chunks, const XcdQueues &xq,
                               unsigned *queue, int waves, int want_density, hipStream_t s)
 {
-    launch_fill_u32(queue, 1, 0u, s);
-    if (nchunks > 0)
+    launch_fill_u32(queue, 8, 0u, s);
+    if (xq.start[8] > 0)
         hipLaunchKernelGGL((k_cells_ref_batched<1, 1, 64>), dim3(waves), dim3(64), 0, s, ins, gps,
-                           outs, rt, chunks, nchunks, queue, want_density);
+                           outs, rt, chunks, xq, queue, want_density);
     hipLaunchKernelGGL((k_cells_ref_slow_batched<8, 2, 512>), dim3(1024), dim3(64), 0, s, ins, rt,
                        want_density);
 }

@@ -15,6 +15,8 @@
 #include "geom.hpp"
 #include "deposit.hpp"
 
+#include <algorithm>
+
 namespace t2 {
 
 enum { EX_DEPOSIT = 0, EX_INCOMPLETE = 2, EX_OUTSIDE = 3, EX_DEGENERATE = 4 };
@@ -308,113 +310,30 @@ void launch_owner(const GridP &gp, const HashP &h, const unsigned *start, const 
 }
 
 // ---------------------------------------------------------------------------
-// deposit status of each cell: complete + Voronoi-vertex bbox inside the
-// data bounds (one wave per cell; star BFS as in the reference-scan kernel)
+// deposit status of each cell: its star is complete and every Voronoi vertex
+// (circumcenter of a star tet) lies inside the data bounds +- 2 FLT_EPSILON
+// (the reference's CellGridPts test, src/dense.cpp:1381-1392, on the exact
+// bbox of the cell).  Both are "for all tets of the star" conditions, so they
+// are decided tet by tet: one lane per tet flags its vertices
+//   - incomplete: a hull face (neighbour -1) makes its three vertices hull
+//     vertices (the star traversal of neighbor_edges would meet it);
+//   - outside: its circumcenter leaves the bounds;
+//   - defined: its circumcenter exists (flat tets of exactly cospherical
+//     points have none and are skipped; a cell with none is degenerate).
+// MIXED decides with the FP32 circumcenter and its error bound (cc.w) and
+// recomputes in FP64 only when the bound straddles a threshold; FP64
+// recomputes every circumcenter.  The flags are plain byte stores of 1.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ int comp4(const int4 &q, int i)
 {
     return i == 0 ? q.x : (i == 1 ? q.y : (i == 2 ? q.z : q.w));
 }
 
-template <int NS>
-__device__ void exact_status_body(const int v, const BlockIn &in, const GridP &gp, int *status,
-                                  const int prec, unsigned *ctrs, int *slow,
-                                  const unsigned slow_cap, const bool can_defer, int4 *s_v,
-                                  int4 *s_n, int *s_h)
+__global__ void k_exact_flags(BlockIn in, GridP gp, int prec, unsigned char *__restrict__ flags,
+                              unsigned *ctrs)
 {
-    constexpr int SMAX = 64 * NS;
-    const int lane = threadIdx.x & 63;
-    const int vt = in.v2t[v];
-    if (vt < 0) {
-        if (lane == 0) status[v] = EX_INCOMPLETE;
-        return;
-    }
-    // the star as a set (the decision below does not depend on its order):
-    // tet ids enter an LDS hash table, the lane whose insert creates an entry
-    // gives the tet the next slot
-    constexpr int HS = 2 * SMAX;
-    int *hk = s_h, *s_tid = s_h + HS;
-    for (int i = lane; i < HS; i += 64) hk[i] = -1;
-    __syncthreads();
-    if (lane == 0) {
-        s_v[0] = in.tets[2 * vt];
-        s_n[0] = in.tets[2 * vt + 1];
-        (void)hash_find<HS>(hk, vt);
-        s_tid[0] = vt;
-    }
-    int S = 1, f0 = 0, f1 = 1;
-    bool hull = false, over = false;
-    __syncthreads();
-    while (f0 < f1) {
-        const int ncand = (f1 - f0) * 4;
-        for (int base = 0; base < ncand; base += 64) {
-            const int key = base + lane;
-            int cand = -1;
-            bool valid = false, hh = false;
-            if (key < ncand) {
-                const int s = f0 + (key >> 2), i = key & 3;
-                if (comp4(s_v[s], i) != v) {
-                    const int ni = comp4(s_n[s], i);
-                    if (ni < 0) hh = true;
-                    else {
-                        valid = true;
-                        cand = ni;
-                    }
-                }
-            }
-            if (__ballot(hh)) {
-                hull = true;
-                break;
-            }
-            bool ins = false;
-            const int e = valid ? hash_find<HS>(hk, cand, ins) : 0;
-            if (__ballot(valid && e < 0)) {   // table full: the star is too large
-                over = true;
-                break;
-            }
-            const unsigned long long L = __ballot(ins);
-            if (ins) {
-                const int slot = S + __popcll(L & ((1ull << lane) - 1ull));
-                if (slot < SMAX) s_tid[slot] = cand;
-            }
-            S += __popcll(L);
-            if (S > SMAX) {
-                over = true;
-                break;
-            }
-        }
-        if (hull || over) break;
-        __syncthreads();
-        for (int b = f1; b < S; b += 64) {
-            const int s = b + lane;
-            if (s < S) {
-                const int id = s_tid[s];
-                s_v[s] = in.tets[2 * id];
-                s_n[s] = in.tets[2 * id + 1];
-            }
-        }
-        __syncthreads();
-        f0 = f1;
-        f1 = S;
-    }
-    if (hull) {
-        if (lane == 0) status[v] = EX_INCOMPLETE;
-        return;
-    }
-    if (over) {
-        if (lane == 0) {
-            if (can_defer) {
-                const unsigned k = atomicAdd(&ctrs[C_SLOW], 1u);
-                if (k < slow_cap) slow[k] = v;
-                else atomicOr(&ctrs[C_ERR], 1u);
-            } else {
-                status[v] = -1;
-                atomicOr(&ctrs[C_ERR], 4u);
-            }
-        }
-        return;
-    }
-    // thresholds as the reference computes them (float)
+    const int nc = in.ncells;
+    unsigned char *inc = flags, *out = flags + nc, *def = flags + 2 * (size_t)nc;
     float lo[3], hi[3];
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
@@ -422,124 +341,89 @@ __device__ void exact_status_body(const int v, const BlockIn &in, const GridP &g
         lo[i] = gp.dmin[i] - e;
         hi[i] = gp.dmax[i] + e;
     }
-    bool decided = false;
-    int st = EX_DEPOSIT;
-    if (prec == 0) {
-        // FP32 circumcenters with their error bounds (cc.w)
-        float mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
-        float err = 0.0f;
-        for (int k = 0; k < NS; ++k) {
-            const int s = k * 64 + lane;
-            if (s < S) {
-                const float4 c = in.cc[s_tid[s]];
-                mn[0] = fminf(mn[0], c.x);
-                mn[1] = fminf(mn[1], c.y);
-                mn[2] = fminf(mn[2], c.z);
-                mx[0] = fmaxf(mx[0], c.x);
-                mx[1] = fmaxf(mx[1], c.y);
-                mx[2] = fmaxf(mx[2], c.z);
-                // NaN or infinite bound: undecidable in float
-                err = (c.w >= 0.0f) ? fmaxf(err, c.w) : INFINITY;
-            }
-        }
-        for (int off = 32; off >= 1; off >>= 1) {
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < in.ntets;
+         t += gridDim.x * blockDim.x) {
+        const int4 vv = in.tets[2 * t];
+        if (vv.x >= nc && vv.y >= nc && vv.z >= nc && vv.w >= nc) continue;   // ghosts only
+        const int4 nb = in.tets[2 * t + 1];
 #pragma unroll
-            for (int i = 0; i < 3; ++i) {
-                mn[i] = fminf(mn[i], __shfl_xor(mn[i], off));
-                mx[i] = fmaxf(mx[i], __shfl_xor(mx[i], off));
-            }
-            err = fmaxf(err, __shfl_xor(err, off));
-        }
-        if (err < 1e-3f) {
-            bool surely_in = true, surely_out = false;
+        for (int i = 0; i < 4; ++i) {
+            if (comp4(nb, i) >= 0) continue;
 #pragma unroll
-            for (int i = 0; i < 3; ++i) {
-                surely_in &= (mn[i] - err > lo[i]) && (mx[i] + err < hi[i]);
-                surely_out |= (mn[i] + err < lo[i]) || (mx[i] - err > hi[i]);
-            }
-            if (surely_in || surely_out) {
-                decided = true;
-                st = surely_out ? EX_OUTSIDE : EX_DEPOSIT;
+            for (int k = 0; k < 4; ++k) {
+                const int u = comp4(vv, k);
+                if (k != i && u < nc) inc[u] = 1;
             }
         }
-        if (!decided && lane == 0) atomicAdd(&ctrs[C_FALLBACK], 1u);
+        bool decided = false, is_out = false, is_def = false;
+        if (prec == 0) {
+            const float4 c = in.cc[t];
+            const float err = c.w;   // NaN or inf: undecidable in float
+            if (err >= 0.0f && err < 1e-3f) {
+                bool surely_in = true, surely_out = false;
+                const float q[3] = {c.x, c.y, c.z};
+#pragma unroll
+                for (int i = 0; i < 3; ++i) {
+                    surely_in &= (q[i] - err > lo[i]) && (q[i] + err < hi[i]);
+                    surely_out |= (q[i] + err < lo[i]) || (q[i] - err > hi[i]);
+                }
+                if (surely_in || surely_out) {
+                    decided = true;
+                    is_def = true;
+                    is_out = surely_out;
+                }
+            }
+            const unsigned long long fb = __ballot(!decided);
+            if ((threadIdx.x & 63) == (unsigned)(__ffsll((unsigned long long)__ballot(1)) - 1) && fb)
+                atomicAdd(&ctrs[C_FALLBACK], (unsigned)__popcll(fb));
+        }
+        if (!decided) {
+            const float4 a = in.pts[vv.x], b = in.pts[vv.y], c = in.pts[vv.z], d = in.pts[vv.w];
+            double ox, oy, oz;
+            circumcenter_f64(a.x, a.y, a.z, b.x, b.y, b.z, c.x, c.y, c.z, d.x, d.y, d.z, ox, oy, oz);
+            if (!(isnan(ox) || isnan(oy) || isnan(oz))) {
+                is_def = true;
+                is_out = ox < (double)lo[0] || oy < (double)lo[1] || oz < (double)lo[2] ||
+                         ox > (double)hi[0] || oy > (double)hi[1] || oz > (double)hi[2];
+            }
+        }
+        if (is_def) {
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const int u = comp4(vv, k);
+                if (u < nc) {
+                    def[u] = 1;
+                    if (is_out) out[u] = 1;
+                }
+            }
+        }
     }
-    if (!decided) {
-        // flat tets have no circumcenter (NaN) and are skipped
-        double mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
-        bool defined = false;
-        for (int k = 0; k < NS; ++k) {
-            const int s = k * 64 + lane;
-            if (s < S) {
-                const int4 vv = s_v[s];
-                const float4 a = in.pts[vv.x], b = in.pts[vv.y], c = in.pts[vv.z], d = in.pts[vv.w];
-                double ox, oy, oz;
-                circumcenter_f64(a.x, a.y, a.z, b.x, b.y, b.z, c.x, c.y, c.z, d.x, d.y, d.z, ox, oy,
-                                 oz);
-                if (isnan(ox) || isnan(oy) || isnan(oz)) continue;
-                defined = true;
-                mn[0] = fmin(mn[0], ox);
-                mn[1] = fmin(mn[1], oy);
-                mn[2] = fmin(mn[2], oz);
-                mx[0] = fmax(mx[0], ox);
-                mx[1] = fmax(mx[1], oy);
-                mx[2] = fmax(mx[2], oz);
-            }
-        }
-        for (int off = 32; off >= 1; off >>= 1) {
-#pragma unroll
-            for (int i = 0; i < 3; ++i) {
-                mn[i] = fmin(mn[i], __shfl_xor(mn[i], off));
-                mx[i] = fmax(mx[i], __shfl_xor(mx[i], off));
-            }
-        }
-        const bool anydef = __ballot(defined) != 0ull;
-        if (!anydef) st = EX_DEGENERATE;
-        else if (mn[0] < (double)lo[0] || mn[1] < (double)lo[1] || mn[2] < (double)lo[2] ||
-                 mx[0] > (double)hi[0] || mx[1] > (double)hi[1] || mx[2] > (double)hi[2])
-            st = EX_OUTSIDE;
-        else st = EX_DEPOSIT;
-    }
-    if (lane == 0) status[v] = st;
 }
 
-__global__ __launch_bounds__(64) void k_exact_status(BlockIn in, GridP gp, int *status, int prec,
-                                                     unsigned *ctrs, int *slow, unsigned slow_cap)
+__global__ void k_exact_classify(const int *__restrict__ v2t, int nc,
+                                 const unsigned char *__restrict__ flags, int *status)
 {
-    __shared__ int4 s_v[64];
-    __shared__ int4 s_n[64];
-    __shared__ int s_h[3 * 64];
-    const int k = xcd_item((int)blockIdx.x, in.ncells);
-    if (k < 0) return;
-    const int v = in.order ? in.order[k] : k;
-    exact_status_body<1>(v, in, gp, status, prec, ctrs, slow, slow_cap, true, s_v, s_n, s_h);
-}
-
-__global__ __launch_bounds__(64) void k_exact_status_slow(BlockIn in, GridP gp, int *status,
-                                                          int prec, unsigned *ctrs, int *slow,
-                                                          unsigned slow_cap)
-{
-    __shared__ int4 s_v[512];
-    __shared__ int4 s_n[512];
-    __shared__ int s_h[3 * 512];
-    unsigned n = ctrs[C_SLOW];
-    if (n > slow_cap) n = slow_cap;
-    for (unsigned k = blockIdx.x; k < n; k += gridDim.x) {
-        exact_status_body<8>(slow[k], in, gp, status, prec, ctrs, slow, slow_cap, false, s_v,
-                             s_n, s_h);
-        __syncthreads();
-    }
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= nc) return;
+    int st;
+    if (v2t[v] < 0 || flags[v]) st = EX_INCOMPLETE;
+    else if (!flags[2 * (size_t)nc + v]) st = EX_DEGENERATE;
+    else if (flags[nc + v]) st = EX_OUTSIDE;
+    else st = EX_DEPOSIT;
+    status[v] = st;
 }
 
 void launch_exact_status(const BlockIn &in, const GridP &gp, int *status, int prec, unsigned *ctrs,
-                         int *slow, unsigned slow_cap, hipStream_t s)
+                         unsigned char *flags, hipStream_t s)
 {
     if (in.ncells <= 0) return;
-    launch_fill_u32(&ctrs[C_SLOW], 1, 0u, s);   // the list is shared with the volume pass
-    hipLaunchKernelGGL(k_exact_status, dim3(xcd_grid(in.ncells)), dim3(64), 0, s, in, gp, status,
-                       prec, ctrs, slow, slow_cap);
-    hipLaunchKernelGGL(k_exact_status_slow, dim3(256), dim3(64), 0, s, in, gp, status, prec, ctrs,
-                       slow, slow_cap);
+    launch_fill_u32((unsigned *)flags, ((size_t)in.ncells * 3 + 3) / 4, 0u, s);
+    if (in.ntets > 0) {
+        const int blocks = std::min(32768, (in.ntets + 255) / 256);
+        hipLaunchKernelGGL(k_exact_flags, dim3(blocks), dim3(256), 0, s, in, gp, prec, flags, ctrs);
+    }
+    hipLaunchKernelGGL(k_exact_classify, dim3((in.ncells + 255) / 256), dim3(256), 0, s, in.v2t,
+                       in.ncells, (const unsigned char *)flags, status);
 }
 
 // ---------------------------------------------------------------------------

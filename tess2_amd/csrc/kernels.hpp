@@ -102,6 +102,13 @@ struct Chunk {
     int pad;
 };
 
+// The chunk list split into 8 contiguous ranges, one per XCD (workgroup b
+// runs on XCD b % 8): waves take chunks of their own range first, so that
+// each XCD's L2 sees one compact part of the data, then help the others.
+struct XcdQueues {
+    int start[9];   // range x = chunks [start[x], start[x+1])
+};
+
 struct BlockIn {
     const float4 *pts;   // xyz + pad
     const int4 *tets;    // 2 int4 per tet: verts, neighbours
@@ -156,8 +163,9 @@ void launch_hash_build(const float4 *pts, int np, const HashP &h, unsigned *star
 void launch_owner(const GridP &gp, const HashP &h, const unsigned *start, const float4 *hp,
                   int *owner, int prec, unsigned *ctrs, const int *status, int *npts,
                   int ncells, hipStream_t s);
+// flags: 3 * ncells bytes of scratch (incomplete, outside, defined)
 void launch_exact_status(const BlockIn &in, const GridP &gp, int *status, int prec,
-                         unsigned *ctrs, int *slow, unsigned slow_cap, hipStream_t s);
+                         unsigned *ctrs, unsigned char *flags, hipStream_t s);
 void launch_exact_tail(const BlockIn &in, const GridP &gp, const int *owner,
                        const int *status_ex, const CellOut &o, int prec, hipStream_t s);
 void launch_fill_u32(unsigned *p, size_t n, unsigned v, hipStream_t s);
@@ -171,7 +179,7 @@ void launch_cells_ref_slow(const BlockIn &in, const GridP &gp, const CellOut &o,
 void launch_cic(const BlockIn &in, const GridP &gp, const CellOut &o, const Router *rt_dev,
                 int self, hipStream_t s);
 void launch_cells_ref_batched(const BlockIn *ins, const GridP *gps, const CellOut *outs,
-                              const Router *rt, const Chunk *chunks, int nchunks,
+                              const Router *rt, const Chunk *chunks, const XcdQueues &xq,
                               unsigned *queue, int waves, int want_density, hipStream_t s);
 // ordered fold of shared grid points and per-block statistics for nb blocks
 // at once (outs/gps: device arrays; the capacities size the grids)

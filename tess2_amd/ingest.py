@@ -148,17 +148,19 @@ def _column_job(args):
         own &= (ov[:, ax] >= lo[ax]) & (ov[:, ax] < hi[ax])
     tv = tv[own]
     cen, r2 = _circumspheres(pts.astype(np.float64), tv)
-    r = np.sqrt(r2)
-    # distance of the centre to the cloud box along each axis
-    gap = np.maximum(np.maximum(cloud_lo - cen, cen - cloud_hi), 0.0)
-    ok = np.isfinite(r)
-    for ax in (0, 1):
-        # half-width along ax of the sphere's intersection with the box
-        rest = r2 - sum(gap[:, o] ** 2 for o in range(3) if o != ax)
-        hw = np.sqrt(np.maximum(rest, 0.0))
-        smin = np.maximum(cen[:, ax] - hw, cloud_lo[ax])
-        smax = np.minimum(cen[:, ax] + hw, cloud_hi[ax])
-        ok &= (smin >= pad_lo[ax]) & (smax <= pad_hi[ax])
+    # flat tets (cospherical points) have no finite sphere and are never kept
+    with np.errstate(invalid="ignore", over="ignore"):
+        r = np.sqrt(r2)
+        # distance of the centre to the cloud box along each axis
+        gap = np.maximum(np.maximum(cloud_lo - cen, cen - cloud_hi), 0.0)
+        ok = np.isfinite(r)
+        for ax in (0, 1):
+            # half-width along ax of the sphere's intersection with the box
+            rest = r2 - sum(gap[:, o] ** 2 for o in range(3) if o != ax)
+            hw = np.sqrt(np.maximum(rest, 0.0))
+            smin = np.maximum(cen[:, ax] - hw, cloud_lo[ax])
+            smax = np.minimum(cen[:, ax] + hw, cloud_hi[ax])
+            ok &= (smin >= pad_lo[ax]) & (smax <= pad_hi[ax])
     return tv, ok, time.time() - t0, sub.shape[0]
 
 
@@ -204,7 +206,8 @@ def tessellate(points: np.ndarray, workers: int = 1, columns: int | None = None,
     one tessellation (tet numbering then follows the columns).  Only tets
     certified Delaunay for the whole set are kept; the first `closed_first`
     points (the original particles of a ghosted block) must end up with
-    closed stars, otherwise the margin doubles and the columns are redone."""
+    closed stars, otherwise the margin doubles and the columns are redone.
+    Degenerate input whose columns cannot be merged falls back to one call."""
     points = np.ascontiguousarray(points, dtype=np.float32)
     n = points.shape[0]
     if workers <= 1 or n < min_points:
@@ -235,7 +238,15 @@ def tessellate(points: np.ndarray, workers: int = 1, columns: int | None = None,
                 dropped += int((~ok).sum())
                 parts.append(tv[ok])
         tv = np.concatenate(parts).astype(np.int32)
-        nb = _rebuild_neighbors(tv)
+        try:
+            nb = _rebuild_neighbors(tv)
+        except RuntimeError:
+            # cospherical point sets have no unique Delaunay triangulation:
+            # columns can split the same empty sphere differently, so the
+            # pieces do not fit; one Qhull call decides consistently
+            if log:
+                log("  columns disagree on degenerate (cospherical) cells; one Qhull call")
+            return qhull_tets(points)
         tets = np.empty((tv.shape[0], 8), np.int32)
         tets[:, :4] = tv
         tets[:, 4:] = nb

@@ -44,7 +44,7 @@ def grf_lattice(n_side: int, rms: float, seed: int = 1, slope: float = -2.0) -> 
     dk = wk * amp
     psi = []
     for K in (KX, KY, KZ):
-        comp = np.fft.irfftn(1j * K / k2 * dk, s=(n, n, n))
+        comp = np.fft.irfftn(1j * K / k2 * dk, s=(n, n, n), axes=(0, 1, 2))
         psi.append(comp)
     psi = np.stack(psi, axis=-1).reshape(-1, 3)
     cur = np.sqrt(np.mean(np.sum(psi * psi, axis=1)))

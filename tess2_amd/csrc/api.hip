@@ -106,7 +106,7 @@ struct tess2_ctx {
     unsigned long long ggen = 0;     // generation the graph was captured at
     bool in_capture = false;
     BlockStats *d_stats = nullptr, *h_stats = nullptr;   // per-block counters, device / pinned
-    double *d_gpart = nullptr;                           // partial grid sums, 256 per block
+    double *d_gpart = nullptr;                           // partial grid sums, 1024 per block
     bool kevents = false;            // the last run recorded the kernel events
     RunArgs desc_args;
     unsigned long long desc_gen = 0;
@@ -530,7 +530,7 @@ extern "C" int tess2_stage(tess2_ctx *c, const tess2_block_t *blocks, int nblock
         if ((r = dalloc(c, &c->d_ins, (size_t)nblocks)) || (r = dalloc(c, &c->d_gps, (size_t)nblocks)) ||
             (r = dalloc(c, &c->d_outs, (size_t)nblocks)) || (r = dalloc(c, &c->d_router, 1)) ||
             (r = dalloc(c, &c->d_queue, 16)) || (r = dalloc(c, &c->d_stats, (size_t)nblocks)) ||
-            (r = dalloc(c, &c->d_gpart, (size_t)nblocks * 256)))
+            (r = dalloc(c, &c->d_gpart, (size_t)nblocks * 1024)))
             return r;
         // counters and statistics of all blocks: one device array, one pinned
         // host copy (so that the copy can sit in a captured graph)

@@ -123,7 +123,7 @@ __global__ void k_cell_stats(const CellOut *__restrict__ outs)
 // per block (blockIdx.y): sum (double) and max of the density grid.  The
 // sum is reduced in a fixed order (lanes, waves, then workgroups in
 // k_grid_final) so that it is the same on every run.
-constexpr int GRID_PARTS = 256;
+constexpr int GRID_PARTS = 1024;
 
 __global__ __launch_bounds__(256) void k_grid_stats(const CellOut *__restrict__ outs,
                                                     double *__restrict__ part)
@@ -151,12 +151,15 @@ __global__ __launch_bounds__(256) void k_grid_stats(const CellOut *__restrict__ 
         part[blockIdx.y * GRID_PARTS + blockIdx.x] = ((s_acc[0] + s_acc[1]) + s_acc[2]) + s_acc[3];
 }
 
-__global__ void k_grid_final(const CellOut *__restrict__ outs, const double *__restrict__ part,
-                             int nparts)
+__global__ __launch_bounds__(256) void k_grid_final(const CellOut *__restrict__ outs,
+                                                    const double *__restrict__ part, int nparts)
 {
+    __shared__ double s[GRID_PARTS];
+    for (int k = threadIdx.x; k < nparts; k += blockDim.x) s[k] = part[blockIdx.x * GRID_PARTS + k];
+    __syncthreads();
     if (threadIdx.x != 0) return;
     double acc = 0.0;
-    for (int k = 0; k < nparts; ++k) acc += part[blockIdx.x * GRID_PARTS + k];
+    for (int k = 0; k < nparts; ++k) acc += s[k];
     outs[blockIdx.x].st->gsum = acc;
 }
 
@@ -164,14 +167,16 @@ void launch_stats(const CellOut *outs, double *part, int nb, int max_cells,
                   unsigned long long max_grid, hipStream_t s)
 {
     if (nb <= 0) return;
-    // about 16 items per lane, at most 256 workgroups per block
-    unsigned bc = (unsigned)std::min<long long>(256, ((long long)max_cells + 4095) / 4096);
-    unsigned bg = (unsigned)std::min<unsigned long long>(GRID_PARTS, (max_grid + 4095) / 4096);
+    // about 16 items per lane, at most GRID_PARTS workgroups per block (a
+    // single block fills the chip; many blocks share it)
+    const long long cap = std::max<long long>(16, GRID_PARTS / std::max(1, nb / 4));
+    unsigned bc = (unsigned)std::min<long long>(cap, ((long long)max_cells + 4095) / 4096);
+    unsigned bg = (unsigned)std::min<unsigned long long>(cap, (max_grid + 4095) / 4096);
     if (bc == 0) bc = 1;
     if (bg == 0) bg = 1;
     hipLaunchKernelGGL(k_cell_stats, dim3(bc, nb), dim3(256), 0, s, outs);
     hipLaunchKernelGGL(k_grid_stats, dim3(bg, nb), dim3(256), 0, s, outs, part);
-    hipLaunchKernelGGL(k_grid_final, dim3(nb), dim3(64), 0, s, outs, (const double *)part,
+    hipLaunchKernelGGL(k_grid_final, dim3(nb), dim3(256), 0, s, outs, (const double *)part,
                        (int)bg);
 }
 

@@ -185,7 +185,7 @@ void launch_cells_ref_batched(const BlockIn *ins, const GridP *gps, const CellOu
 // at once (outs/gps: device arrays; the capacities size the grids)
 void launch_fold(const CellOut *outs, const GridP *gps, int nb, unsigned max_ovf_cap,
                  const unsigned max_fold_cap[3], hipStream_t s);
-// part: nb * 256 doubles of scratch for the fixed-order grid sums
+// part: nb * 1024 doubles of scratch for the fixed-order grid sums
 void launch_stats(const CellOut *outs, double *part, int nb, int max_cells,
                   unsigned long long max_grid, hipStream_t s);
 

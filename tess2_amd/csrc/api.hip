@@ -432,12 +432,16 @@ extern "C" int tess2_stage(tess2_ctx *c, const tess2_block_t *blocks, int nblock
         int r = validate(c, blocks[i]);
         if (r) return r;
     }
+    // deposit keys: cell << 1 | z bit for own cells; 1 << 31 | rank << 24 |
+    // cell << 1 | z bit for points received from another block
+    for (int i = 0; i < nblocks; ++i)
+        if (blocks[i].num_orig_particles >= (1 << 30))
+            return fail(c, TESS2_E_ARG, "at most 2^30 - 1 cells per block");
     if (nblocks > 1) {
-        // deposit keys hold (source rank, cell) in 7 + 24 bits
         if (nblocks > 127) return fail(c, TESS2_E_ARG, "at most 127 blocks per context");
         for (int i = 0; i < nblocks; ++i) {
-            if (blocks[i].num_orig_particles >= (1 << 24))
-                return fail(c, TESS2_E_ARG, "multi-block runs need < 2^24 cells per block");
+            if (blocks[i].num_orig_particles >= (1 << 23))
+                return fail(c, TESS2_E_ARG, "multi-block runs need < 2^23 cells per block");
             if (i > 0 && blocks[i].gid <= blocks[i - 1].gid)
                 return fail(c, TESS2_E_ARG, "blocks must be given in ascending gid order");
         }

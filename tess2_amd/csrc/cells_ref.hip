@@ -474,7 +474,7 @@ __device__ void cell_ref_body(const int v, const BlockIn &in, const GridP &gp,
                             for (int b0 = 0; b0 < len; b0 += 64)
                                 if (b0 + lane < len)
                                     deposit_point(cmi0 + min_xi + b0 + lane, cmi1 + yi, cmi2 + zi,
-                                                  mpp, v, gp, o, rt, self);
+                                                  mpp, cell_key(v, 0), gp, o, rt, self);
                         }
                         if (yi == y_start) first_x = (min_xi + max_xi) / 2;
                         if (!border_found) {
@@ -513,8 +513,8 @@ __device__ void cell_ref_body(const int v, const BlockIn &in, const GridP &gp,
                     if (p >= runs[k].w) r = k;
                 if (p < tot) {
                     const int4 rr = runs[r];
-                    deposit_point(cmi0 + rr.z + (p - rr.w), cmi1 + rr.y, cmi2 + rr.x, mpp, v,
-                                  gp, o, rt, self);
+                    deposit_point(cmi0 + rr.z + (p - rr.w), cmi1 + rr.y, cmi2 + rr.x, mpp,
+                                  cell_key(v, 0), gp, o, rt, self);
                 }
             }
             if (lane == 0) {
@@ -531,7 +531,7 @@ __device__ void cell_ref_body(const int v, const BlockIn &in, const GridP &gp,
                 for (int k = 1; k < 8; ++k)
                     if (lane == k) m = m8[k];
                 deposit_point(mn[0] + (lane & 1), mn[1] + ((lane >> 1) & 1), mn[2] + (lane >> 2),
-                              m, v, gp, o, rt, self);
+                              m, cell_key(v, gp.project ? (lane >> 2) : 0), gp, o, rt, self);
             }
             if (lane == 0) {
                 o.status[v] = ST_CIC;
@@ -699,8 +699,8 @@ __global__ void k_cic(BlockIn in, GridP gp, CellOut o, const Router *rt, int sel
     float m8[8];
     cic_weights(p.x, p.y, p.z, gp.mass, gp, mn, m8);
     for (int k = 0; k < 8; ++k)
-        deposit_point(mn[0] + (k & 1), mn[1] + ((k >> 1) & 1), mn[2] + (k >> 2), m8[k], v,
-                      gp, o, rt, self);
+        deposit_point(mn[0] + (k & 1), mn[1] + ((k >> 1) & 1), mn[2] + (k >> 2), m8[k],
+                      cell_key(v, gp.project ? (k >> 2) : 0), gp, o, rt, self);
     o.status[v] = ST_CIC;
     o.npts[v] = 8;
 }

@@ -10,10 +10,21 @@ namespace t2 {
 
 // ---------------------------------------------------------------------------
 // ordered-accumulation protocol on one block grid point: the first
-// contribution writes the density; later ones join the point's list.  key
-// orders the fold: a block's own cells by cell id (phase 0), then points
-// received from other blocks by (source rank, cell) (phase 1, bit 31).
+// contribution writes the density; later ones join the point's list.  The
+// key orders the fold: a block's own cells by cell id, then points received
+// from other blocks by (source rank, cell) (bit 31 set).  Its low bit is the
+// z offset of a CIC corner in the 2-D projection, where corners z and z+1 of
+// one cell land on the same (x, y) point: the reference adds them in corner
+// order (z first).  Scan points of one cell that share (x, y) all carry the
+// same mass, so their order cannot change the sum.
+//   own:      cell << 1 | dz                         (cells < 2^30)
+//   received: 1 << 31 | rank << 24 | cell << 1 | dz  (cells < 2^23, rank < 127)
 // ---------------------------------------------------------------------------
+__device__ __forceinline__ unsigned cell_key(int cell, int dz)
+{
+    return ((unsigned)cell << 1) | (unsigned)dz;
+}
+
 __device__ __forceinline__ void push_point(long long lidx, float m, unsigned key, float div,
                                            const CellOut &o)
 {
@@ -45,7 +56,7 @@ __device__ __forceinline__ long long block_lidx(int gx, int gy, int gz, const Gr
 // the block whose grid holds its global index, as received mass
 // (src/dense.cpp:298-305 -> recvd_pts :166-202), or is dropped and counted.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ void deposit_point(int gx, int gy, int gz, float m, int cell,
+__device__ __forceinline__ void deposit_point(int gx, int gy, int gz, float m, unsigned ckey,
                                               const GridP &gp, const CellOut &o,
                                               const Router *rt = nullptr, int self = 0)
 {
@@ -55,7 +66,7 @@ __device__ __forceinline__ void deposit_point(int gx, int gy, int gz, float m, i
     const long long lidx = block_lidx(gx, gy, gz, gp);
     if (lidx >= 0 && px >= gp.bmin[0] && px <= gp.bmax[0] && py >= gp.bmin[1] &&
         py <= gp.bmax[1] && pz >= gp.bmin[2] && pz <= gp.bmax[2]) {
-        push_point(lidx, m, (unsigned)cell, gp.div, o);
+        push_point(lidx, m, ckey, gp.div, o);
         return;
     }
     if (rt) {
@@ -63,7 +74,7 @@ __device__ __forceinline__ void deposit_point(int gx, int gy, int gz, float m, i
             if (j == self) continue;
             const long long lj = block_lidx(gx, gy, gz, rt->gps[j]);
             if (lj >= 0) {
-                const unsigned key = 0x80000000u | ((unsigned)self << 24) | (unsigned)cell;
+                const unsigned key = 0x80000000u | ((unsigned)self << 24) | ckey;
                 push_point(lj, m, key, rt->gps[j].div, rt->outs[j]);
                 atomicAdd(&o.ctrs[C_ROUTED], 1u);
                 return;

@@ -427,17 +427,9 @@ void launch_exact_status(const BlockIn &in, const GridP &gp, int *status, int pr
 }
 
 // ---------------------------------------------------------------------------
-// counts, density, CIC of deposited cells without points
+// density, CIC of deposited cells without points (the point counts come from
+// k_owner)
 // ---------------------------------------------------------------------------
-__global__ void k_exact_count(const int *__restrict__ owner, size_t n, const int *__restrict__ status,
-                              int ncells, int *__restrict__ npts)
-{
-    for (size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x; g < n;
-         g += (size_t)gridDim.x * blockDim.x) {
-        const int o = owner[g];
-        if (o >= 0 && o < ncells && status[o] == EX_DEPOSIT) atomicAdd(&npts[o], 1);
-    }
-}
 
 __global__ void k_exact_density(GridP gp, const int *__restrict__ owner, const int *__restrict__ status,
                                 const int *__restrict__ npts, int ncells, CellOut o, int prec)
@@ -452,8 +444,8 @@ __global__ void k_exact_density(GridP gp, const int *__restrict__ owner, const i
         const int lz = (int)(g / ((size_t)gp.bnum_idx[0] * gp.bnum_idx[1]));
         const int N = npts[ow];
         const float m = prec == 0 ? gp.mass / (float)N : (float)((double)gp.mass / (double)N);
-        deposit_point(gp.bmin_idx[0] + lx, gp.bmin_idx[1] + ly, gp.bmin_idx[2] + lz, m, ow, gp,
-                         o);
+        deposit_point(gp.bmin_idx[0] + lx, gp.bmin_idx[1] + ly, gp.bmin_idx[2] + lz, m,
+                      cell_key(ow, 0), gp, o);
     }
 }
 
@@ -475,8 +467,8 @@ __global__ void k_exact_cic(BlockIn in, GridP gp, const int *__restrict__ status
             float m8[8];
             cic_weights(p.x, p.y, p.z, gp.mass, gp, mn, m8);
             for (int k = 0; k < 8; ++k)
-                deposit_point(mn[0] + (k & 1), mn[1] + ((k >> 1) & 1), mn[2] + (k >> 2), m8[k], v,
-                                 gp, o);
+                deposit_point(mn[0] + (k & 1), mn[1] + ((k >> 1) & 1), mn[2] + (k >> 2), m8[k],
+                              cell_key(v, 0), gp, o);
             out_status = 1;
             out_n = 8;
         }

@@ -30,7 +30,7 @@ enum {
     C_SLOW = 2,       // cells queued for the large-capacity pass
     C_ERR = 3,        // bit 0: deferred-cell list full; bit 2: cell beyond all capacities;
                       // bit 3: a shared point beyond the fold's capacities
-    C_OFFGRID = 4,    // (unused)
+    C_FREE4 = 4,      // (free)
     C_FALLBACK = 5,   // FP64 re-checks of the exact assignment
     C_ROUTED = 6,     // points deposited into another block's grid
     C_SEG = 7,        // contributions gathered into per-point segments for the fold

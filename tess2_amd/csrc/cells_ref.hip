@@ -426,23 +426,33 @@ __device__ void cell_ref_body(const int v, const BlockIn &in, const GridP &gp,
             for (int zi = 0; zi < n2; zi++) {
                 bool border_found = false, z_step_done = false;
                 for (int yi = y_start; yi < n1; yi++) {
-                    const bool x_in = PROBE(x_left, yi, zi);
+                    // the left walk starts where x_in was taken: its first
+                    // verdict is x_in (PtInCell is pure), each later one is
+                    // taken after the step
+                    const int x0 = x_left;
+                    const bool x_in = PROBE(x0, yi, zi);
                     int min_xi = n0 - 1, max_xi = 0;
-                    for (int xi = x_left; xi >= 0 && xi < n0;) {
-                        if (PROBE(xi, yi, zi)) {
-                            if (xi < min_xi) min_xi = xi;
-                            if (xi > max_xi) max_xi = xi;
-                            if (x_in) xi--;
-                            else {
-                                x_left = xi;
-                                break;
+                    {
+                        int xi = x0;
+                        bool in_xi = x_in;
+                        while (xi >= 0 && xi < n0) {
+                            if (in_xi) {
+                                if (xi < min_xi) min_xi = xi;
+                                if (xi > max_xi) max_xi = xi;
+                                if (x_in) xi--;
+                                else {
+                                    x_left = xi;
+                                    break;
+                                }
+                            } else {
+                                if (!x_in) xi++;
+                                else {
+                                    x_left = xi;
+                                    break;
+                                }
                             }
-                        } else {
-                            if (!x_in) xi++;
-                            else {
-                                x_left = xi;
-                                break;
-                            }
+                            if (xi < 0 || xi >= n0) break;
+                            in_xi = PROBE(xi, yi, zi);
                         }
                     }
                     for (int xi = x_right; xi >= 0 && xi < n0;) {

@@ -922,7 +922,8 @@ extern "C" int tess2_sync(tess2_ctx *c, tess2_stats_t *st)
             st->cells_degenerate += (long long)d.hm->cstat[4];
             st->points_offblock += d.hm->ctrs[C_OFFBLOCK];
             st->points_routed += d.hm->ctrs[C_ROUTED];
-            st->points_deposited += (long long)d.hm->cstat[5] - d.hm->ctrs[C_OFFBLOCK];
+            st->points_deposited += (long long)d.hm->cstat[5] - d.hm->ctrs[C_OFFBLOCK] -
+                                    d.hm->ctrs[C_ROUTED];
             st->points_shared += d.hm->ctrs[C_OVF];
             st->fp64_fallbacks += d.hm->ctrs[C_FALLBACK];
             st->cells_deferred += d.hm->ctrs[C_SLOW];

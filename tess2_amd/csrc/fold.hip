@@ -39,13 +39,14 @@ __device__ __forceinline__ unsigned fold_capacity(const CellOut &o, int cls)
 }
 
 // 1. one segment per shared point; slot and segment reservations are
-// aggregated per wave (one atomic per wave and counter)
+// aggregated per workgroup (one atomic per workgroup and counter per pass)
 __global__ __launch_bounds__(256) void k_fold_gather(const CellOut *__restrict__ outs)
 {
+    __shared__ unsigned s_cnt[4][4];   // per wave: points of class 0..2, segment total
     const CellOut &o = outs[blockIdx.y];
     if (fold_overflowed(o)) return;
     const unsigned n = o.ctrs[C_OVF];
-    const int lane = threadIdx.x & 63;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const unsigned long long below = (1ull << lane) - 1ull;
     for (unsigned base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
         const unsigned e = base + threadIdx.x;
@@ -60,40 +61,50 @@ __global__ __launch_bounds__(256) void k_fold_gather(const CellOut *__restrict__
                 cls = cnt <= FOLD_SMALL_MAX ? 0 : (cnt <= FOLD_MID_MAX ? 1 : 2);
             }
         }
-        unsigned idx = 0;
-#pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            const unsigned long long m = __ballot(mine && cls == k);
-            if (m) {
-                const int leader = __ffsll((unsigned long long)m) - 1;
-                unsigned b0 = 0;
-                if (lane == leader) b0 = atomicAdd(&o.ctrs[C_FOLD0 + k], (unsigned)__popcll(m));
-                b0 = __shfl(b0, leader);
-                if (mine && cls == k) idx = b0 + (unsigned)__popcll(m & below);
-            }
-        }
-        unsigned incl = cnt;   // inclusive prefix sum of the segment sizes
+        const unsigned long long m0 = __ballot(mine && cls == 0);
+        const unsigned long long m1 = __ballot(mine && cls == 1);
+        const unsigned long long m2 = __ballot(mine && cls == 2);
+        unsigned incl = cnt;   // inclusive prefix sum of the segment sizes in the wave
         for (int off = 1; off < 64; off <<= 1) {
             const unsigned t = __shfl_up(incl, off);
             if (lane >= off) incl += t;
         }
-        const unsigned total = __shfl(incl, 63);
-        unsigned segb = 0;
-        if (total) {
-            if (lane == 0) segb = atomicAdd(&o.ctrs[C_SEG], total);
-            segb = __shfl(segb, 0);
+        if (lane == 63) {
+            s_cnt[w][0] = (unsigned)__popcll(m0);
+            s_cnt[w][1] = (unsigned)__popcll(m1);
+            s_cnt[w][2] = (unsigned)__popcll(m2);
+            s_cnt[w][3] = incl;
         }
-        if (!mine) continue;
-        const unsigned off = segb + incl - cnt;
-        if (idx >= fold_capacity(o, cls) || (unsigned long long)off + cnt > 2ull * o.ovf_cap ||
-            idx >= (1u << 28)) {
-            atomicOr(&o.ctrs[C_ERR], 8u);
-            o.head[lidx] = -(1 + 3);              // class 3: skipped by the scatter
-            continue;
+        __syncthreads();
+        if (threadIdx.x < 4) {   // one reservation per counter, then per-wave bases
+            const int k = threadIdx.x;
+            const int nw = (int)(blockDim.x >> 6);
+            unsigned sum = 0;
+            for (int i = 0; i < nw; ++i) sum += s_cnt[i][k];
+            unsigned b0 = 0;
+            if (sum) b0 = atomicAdd(&o.ctrs[k < 3 ? C_FOLD0 + k : C_SEG], sum);
+            for (int i = 0; i < nw; ++i) {
+                const unsigned c = s_cnt[i][k];
+                s_cnt[i][k] = b0;
+                b0 += c;
+            }
         }
-        fold_list(o, cls)[idx] = make_int4(lidx, (int)off, (int)cnt, 0);
-        o.seg[off + cnt - 1] = o.first[lidx];
-        o.head[lidx] = -(1 + (int)(idx * 4 + cls));
+        __syncthreads();
+        if (mine) {
+            const unsigned long long m = cls == 0 ? m0 : (cls == 1 ? m1 : m2);
+            const unsigned idx = s_cnt[w][cls] + (unsigned)__popcll(m & below);
+            const unsigned off = s_cnt[w][3] + incl - cnt;
+            if (idx >= fold_capacity(o, cls) || (unsigned long long)off + cnt > 2ull * o.ovf_cap ||
+                idx >= (1u << 28)) {
+                atomicOr(&o.ctrs[C_ERR], 8u);
+                o.head[lidx] = -(1 + 3);              // class 3: skipped by the scatter
+            } else {
+                fold_list(o, cls)[idx] = make_int4(lidx, (int)off, (int)cnt, 0);
+                o.seg[off + cnt - 1] = o.first[lidx];
+                o.head[lidx] = -(1 + (int)(idx * 4 + cls));
+            }
+        }
+        __syncthreads();   // s_cnt is reused by the next pass
     }
 }
 

@@ -142,17 +142,54 @@ __device__ __forceinline__ bool better64(double d, int id, double bd, int bid)
     return d < bd || (d == bd && id < bid);
 }
 
-// search in double; returns the owner id
-__device__ int nearest_f64(float gx, float gy, float gz, const HashP &h,
-                           const unsigned *__restrict__ start, const float4 *__restrict__ hp)
+// The buckets of shell r around bucket c (Chebyshev distance r), as runs of
+// the bucket-sorted particle array: buckets are numbered x fastest, so the
+// buckets of a row that lies on the shell's z or y faces form one run
+// [start[b0], start[b1 + 1]); an inner row contributes its two end buckets.
+// Each (dz, dy) row gives at most two runs, visited in order by the caller.
+struct ShellRow {
+    unsigned a0, a1, b0, b1;   // runs [a0, a1) and [b0, b1)
+};
+
+__device__ __forceinline__ ShellRow shell_row(int r, int dz, int dy, int y, int z, int cx,
+                                              const HashP &h, const unsigned *__restrict__ start)
 {
-    int c[3];
+    const int row = (z * h.dims[1] + y) * h.dims[0];
+    const bool face = (dz == -r || dz == r || dy == -r || dy == r);
+    ShellRow s{0u, 0u, 0u, 0u};
+    if (face) {
+        const int xa = max(cx - r, 0), xb = min(cx + r, h.dims[0] - 1);
+        s.a0 = start[row + xa];
+        s.a1 = start[row + xb + 1];
+    } else {
+        if (cx - r >= 0) {
+            s.a0 = start[row + cx - r];
+            s.a1 = start[row + cx - r + 1];
+        }
+        if (cx + r < h.dims[0]) {
+            s.b0 = start[row + cx + r];
+            s.b1 = start[row + cx + r + 1];
+        }
+    }
+    return s;
+}
+
+__device__ __forceinline__ void home_bucket(float gx, float gy, float gz, const HashP &h, int c[3])
+{
     const float q[3] = {gx, gy, gz};
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
         int ci = (int)floorf((q[i] - h.lo[i]) * h.inv);
         c[i] = ci < 0 ? 0 : (ci >= h.dims[i] ? h.dims[i] - 1 : ci);
     }
+}
+
+// search in double; returns the owner id
+__device__ int nearest_f64(float gx, float gy, float gz, const HashP &h,
+                           const unsigned *__restrict__ start, const float4 *__restrict__ hp)
+{
+    int c[3];
+    home_bucket(gx, gy, gz, h, c);
     double best = 1e300;
     int bid = 0x7fffffff;
     const int rmax = h.dims[0] + h.dims[1] + h.dims[2];
@@ -163,13 +200,10 @@ __device__ int nearest_f64(float gx, float gy, float gz, const HashP &h,
             for (int dy = -r; dy <= r; ++dy) {
                 const int y = c[1] + dy;
                 if (y < 0 || y >= h.dims[1]) continue;
-                const bool face = (dz == -r || dz == r || dy == -r || dy == r);
-                const int step = face ? 1 : 2 * r;
-                for (int dx = -r; dx <= r; dx += (step > 0 ? step : 1)) {
-                    const int x = c[0] + dx;
-                    if (x < 0 || x >= h.dims[0]) continue;
-                    const int b = (z * h.dims[1] + y) * h.dims[0] + x;
-                    for (unsigned s = start[b]; s < start[b + 1]; ++s) {
+                const ShellRow sr = shell_row(r, dz, dy, y, z, c[0], h, start);
+                for (int k = 0; k < 2; ++k) {
+                    const unsigned s1 = k ? sr.b1 : sr.a1;
+                    for (unsigned s = k ? sr.b0 : sr.a0; s < s1; ++s) {
                         const float4 p = hp[s];
                         const double ex = (double)gx - (double)p.x;
                         const double ey = (double)gy - (double)p.y;
@@ -198,12 +232,7 @@ __device__ int nearest_f32(float gx, float gy, float gz, const HashP &h,
                            const unsigned *__restrict__ start, const float4 *__restrict__ hp)
 {
     int c[3];
-    const float q[3] = {gx, gy, gz};
-#pragma unroll
-    for (int i = 0; i < 3; ++i) {
-        int ci = (int)floorf((q[i] - h.lo[i]) * h.inv);
-        c[i] = ci < 0 ? 0 : (ci >= h.dims[i] ? h.dims[i] - 1 : ci);
-    }
+    home_bucket(gx, gy, gz, h, c);
     float best = INFINITY, second = INFINITY;
     int bid = -1;
     const int rmax = h.dims[0] + h.dims[1] + h.dims[2];
@@ -214,13 +243,10 @@ __device__ int nearest_f32(float gx, float gy, float gz, const HashP &h,
             for (int dy = -r; dy <= r; ++dy) {
                 const int y = c[1] + dy;
                 if (y < 0 || y >= h.dims[1]) continue;
-                const bool face = (dz == -r || dz == r || dy == -r || dy == r);
-                const int step = face ? 1 : 2 * r;
-                for (int dx = -r; dx <= r; dx += (step > 0 ? step : 1)) {
-                    const int x = c[0] + dx;
-                    if (x < 0 || x >= h.dims[0]) continue;
-                    const int b = (z * h.dims[1] + y) * h.dims[0] + x;
-                    for (unsigned s = start[b]; s < start[b + 1]; ++s) {
+                const ShellRow sr = shell_row(r, dz, dy, y, z, c[0], h, start);
+                for (int k = 0; k < 2; ++k) {
+                    const unsigned s1 = k ? sr.b1 : sr.a1;
+                    for (unsigned s = k ? sr.b0 : sr.a0; s < s1; ++s) {
                         const float4 p = hp[s];
                         const float ex = gx - p.x, ey = gy - p.y, ez = gz - p.z;
                         const float d = ex * ex + ey * ey + ez * ez;

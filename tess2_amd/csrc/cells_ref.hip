@@ -28,6 +28,7 @@
 #include "kernels.hpp"
 #include "geom.hpp"
 #include "deposit.hpp"
+#include "scan_bits.hpp"
 
 namespace t2 {
 
@@ -380,13 +381,62 @@ __device__ void cell_ref_body(const int v, const BlockIn &in, const GridP &gp,
         const int cmi0 = phys2idx1(mnx, gp.step[0], gp.gmin[0]);
         const int cmi1 = phys2idx1(mny, gp.step[1], gp.gmin[1]);
         const int cmi2 = phys2idx1(mnz, gp.step[2], gp.gmin[2]);
-        const int n0 = phys2idx1(mxx, gp.step[0], gp.gmin[0]) - cmi0 + 1;
-        const int n1 = phys2idx1(mxy, gp.step[1], gp.gmin[1]) - cmi1 + 1;
-        const int n2 = phys2idx1(mxz, gp.step[2], gp.gmin[2]) - cmi2 + 1;
+        // every lane holds the same reduced bbox; the box dimensions are
+        // made wave-uniform so that the scan's control runs on the scalar unit
+        const int n0 = __builtin_amdgcn_readfirstlane(phys2idx1(mxx, gp.step[0], gp.gmin[0]) - cmi0 + 1);
+        const int n1 = __builtin_amdgcn_readfirstlane(phys2idx1(mxy, gp.step[1], gp.gmin[1]) - cmi1 + 1);
+        const int n2 = __builtin_amdgcn_readfirstlane(phys2idx1(mxz, gp.step[2], gp.gmin[2]) - cmi2 + 1);
         const float p0 = idx2phys1(cmi0, gp.step[0], gp.gmin[0]);
         const float p1 = idx2phys1(cmi1, gp.step[1], gp.gmin[1]);
         const float p2 = idx2phys1(cmi2, gp.step[2], gp.gmin[2]);
         const float eps = gp.eps;
+
+        // PtInCell for the whole cell box at once when it has at most 64
+        // points with each axis padded to a power of two (x to at most 32):
+        // one lane per point, the faces broadcast from LDS, the verdicts
+        // ballotted into one wave-uniform mask.  A verdict is the same function
+        // of the same float operations as a face-parallel probe's; the scan
+        // then walks each line with bit scans and reads single bits where it
+        // probes.  Larger boxes probe face-parallel as they go.
+        const int lb0 = n0 > 1 ? 32 - __clz(n0 - 1) : 0;
+        const int lb1 = n1 > 1 ? 32 - __clz(n1 - 1) : 0;
+        const int lb2 = n2 > 1 ? 32 - __clz(n2 - 1) : 0;
+        const int lb01 = lb0 + lb1;
+        const bool use_mask = lb0 <= 5 && lb01 + lb2 <= 6;
+        unsigned long long mk = 0ull;
+        if (use_mask) {
+            float4 *s_fa = s_cc;   // star records and circumcenters are done
+            float2 *s_fb = reinterpret_cast<float2 *>(s_v);
+            __syncthreads();
+#pragma unroll
+            for (int j = 0; j < NF; ++j) {
+                const int f = j * 64 + lane;
+                if (f < F) {
+                    s_fa[f] = make_float4(fnx[j], fny[j], fnz[j], ffx[j]);
+                    s_fb[f] = make_float2(ffy[j], ffz[j]);
+                }
+            }
+            __syncthreads();
+            const int x = lane & ((1 << lb0) - 1), y = (lane >> lb0) & ((1 << lb1) - 1);
+            const int z = lane >> lb01;
+            const float qx = p0 + (float)x * gp.step[0];
+            const float qy = p1 + (float)y * gp.step[1];
+            const float qz = p2 + (float)z * gp.step[2];
+            // largest and smallest significant distance: a face with d >= 0
+            // exists iff hi >= 0, one with d < 0 iff lo < 0
+            float hi = -INFINITY, lo = INFINITY;
+#pragma unroll 4
+            for (int k = 0; k < F; ++k) {
+                const float4 a = s_fa[k];
+                const float2 b = s_fb[k];
+                const float d = a.x * (qx - a.w) + a.y * (qy - b.x) + a.z * (qz - b.y);
+                const bool sig = fabsf(d) > eps;
+                hi = fmaxf(hi, sig ? d : -INFINITY);
+                lo = fminf(lo, sig ? d : INFINITY);
+            }
+            const bool in_cell = !(hi >= 0.0f && lo < 0.0f);
+            mk = __ballot(x < n0 && y < n1 && z < n2 && in_cell);
+        }
 
         // PtInCell over all faces at once
         auto inside = [&](float qx, float qy, float qz) -> bool {
@@ -406,8 +456,9 @@ __device__ void cell_ref_body(const int v, const BlockIn &in, const GridP &gp,
         };
 
 #define PROBE(xi, yi, zi)                                                                     \
-    inside(p0 + (float)(xi) * gp.step[0], p1 + (float)(yi) * gp.step[1],                       \
-           p2 + (float)(zi) * gp.step[2])
+    (use_mask ? (bool)((mk >> ((xi) | ((yi) << lb0) | ((zi) << lb01))) & 1ull)                 \
+              : inside(p0 + (float)(xi) * gp.step[0], p1 + (float)(yi) * gp.step[1],             \
+                       p2 + (float)(zi) * gp.step[2]))
 
         // ---- 5. scanline search (src/dense.cpp:1486-1676) --------------------
         // Pass 0 counts the points and keeps the runs (one per z-y line) in
@@ -426,12 +477,17 @@ __device__ void cell_ref_body(const int v, const BlockIn &in, const GridP &gp,
             for (int zi = 0; zi < n2; zi++) {
                 bool border_found = false, z_step_done = false;
                 for (int yi = y_start; yi < n1; yi++) {
+                    int min_xi = n0 - 1, max_xi = 0;
+                    if (use_mask) {
+                        const int base = (yi << lb0) | (zi << lb01);
+                        const unsigned long long row = (mk >> base) & ((1ull << n0) - 1ull);
+                        walk_row_bits(row, n0, x_left, x_right, min_xi, max_xi);
+                    } else {
                     // the left walk starts where x_in was taken: its first
                     // verdict is x_in (PtInCell is pure), each later one is
                     // taken after the step
                     const int x0 = x_left;
                     const bool x_in = PROBE(x0, yi, zi);
-                    int min_xi = n0 - 1, max_xi = 0;
                     {
                         int xi = x0;
                         bool in_xi = x_in;
@@ -471,6 +527,7 @@ __device__ void cell_ref_body(const int v, const BlockIn &in, const GridP &gp,
                                 break;
                             }
                         }
+                    }
                     }
                     int first_y = y_start;
                     if (min_xi <= max_xi) {

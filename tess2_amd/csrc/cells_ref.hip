@@ -111,8 +111,8 @@ __device__ void cell_ref_body(const int v, const BlockIn &in, const GridP &gp,
             if (key < ncand) {
                 s = f0 + (key >> 2);
                 i = key & 3;
-                const int vi = comp(s_v[s], i);
-                const int ni = comp(s_u[s], i);
+                const int vi = reinterpret_cast<const int *>(s_v)[key + 4 * f0];
+                const int ni = reinterpret_cast<const int *>(s_u)[key + 4 * f0];
                 if (vi != v) {
                     if (ni < 0) h = true;
                     else {
@@ -188,7 +188,7 @@ __device__ void cell_ref_body(const int v, const BlockIn &in, const GridP &gp,
             int u = -1;
             bool valid = false;
             if (e < nent) {
-                u = comp(s_v[e >> 2], e & 3);
+                u = reinterpret_cast<const int *>(s_v)[e];
                 valid = (u != v);
             }
             const int hx = valid ? hash_find<HS>(hk, u) : 0;

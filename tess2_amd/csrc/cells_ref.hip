@@ -244,20 +244,15 @@ __device__ void cell_ref_body(const int v, const BlockIn &in, const GridP &gp,
                 float cx = c0.x, cy = c0.y, cz = c0.z;
                 float nx = 0.0f, ny = 0.0f, nz = 0.0f;
                 float area = 0.0f;
-                if (c0.x < mnx) mnx = c0.x;
-                if (c0.y < mny) mny = c0.y;
-                if (c0.z < mnz) mnz = c0.z;
-                if (c0.x > mxx) mxx = c0.x;
-                if (c0.y > mxy) mxy = c0.y;
-                if (c0.z > mxz) mxz = c0.z;
                 int t = us, L = 1;
+                // the vertex at wi: the ring walk crosses the face opposite it
+                int behind = comp(tv, wi);
                 for (;;) {
-                    // circulate_next (src/tet.cpp:185-215)
-                    int nvid = -1;
-                    for (int i = 3; i >= 0; --i) {
-                        const int w = comp(tv, i);
-                        if (i != wi && w != v && w != u) nvid = w;
-                    }
+                    // circulate_next (src/tet.cpp:185-215): the next vertex is
+                    // the one of the tet's four that is not v, u or behind
+                    const int nvid = (int)((unsigned)tv.x + (unsigned)tv.y + (unsigned)tv.z +
+                                           (unsigned)tv.w - (unsigned)v - (unsigned)u -
+                                           (unsigned)behind);
                     const int nt = s_ns[t * 4 + wi];
                     if (nt == us || nt < 0) break;
                     const int4 nv4 = s_v[nt];
@@ -277,18 +272,13 @@ __device__ void cell_ref_body(const int v, const BlockIn &in, const GridP &gp,
                         const float pz = abx * acy - aby * acx;
                         area = (float)((double)area + sqrt((double)nrm3(px, py, pz)) / 2.0);
                     }
-                    if (c.x < mnx) mnx = c.x;
-                    if (c.y < mny) mny = c.y;
-                    if (c.z < mnz) mnz = c.z;
-                    if (c.x > mxx) mxx = c.x;
-                    if (c.y > mxy) mxy = c.y;
-                    if (c.z > mxz) mxz = c.z;
                     cx = c.x;
                     cy = c.y;
                     cz = c.z;
                     t = nt;
                     tv = nv4;
                     wi = nwi;
+                    behind = nvid;
                     L++;
                     if (nwi < 0 || L > LMAX) {
                         ring_bad = (L > LMAX);
@@ -322,6 +312,19 @@ __device__ void cell_ref_body(const int v, const BlockIn &in, const GridP &gp,
             }
         }
         if (__ballot(ring_bad)) goto defer;
+        // cell bbox (src/dense.cpp:701-714): every ring vertex is the
+        // circumcenter of a star tet and every star tet lies on the rings of
+        // its three edges from the site, so the bbox of the rings is the bbox
+        // of the star's circumcenters -- one lane per tet here
+        for (int k = lane; k < S; k += 64) {
+            const float4 c = s_cc[k];
+            if (c.x < mnx) mnx = c.x;
+            if (c.y < mny) mny = c.y;
+            if (c.z < mnz) mnz = c.z;
+            if (c.x > mxx) mxx = c.x;
+            if (c.y > mxy) mxy = c.y;
+            if (c.z > mxz) mxz = c.z;
+        }
 
         // ---- 3. volume: faces summed in neighbour order ----------------------
         if (o.vol) {

@@ -42,6 +42,26 @@ __device__ __forceinline__ float rdlf(float x, int l)
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), l));
 }
 
+// Correctly rounded square root of a double that holds a float value (so
+// never below 2^-149 unless zero): Goldschmidt's iteration from the hardware
+// reciprocal square root with two Newton corrections of the root, as the
+// compiler expands sqrt(double), without the rescaling of inputs below
+// 2^-767 that such values never need.  Zero and +inf return themselves.
+__device__ __forceinline__ double sqrt_of_float(double x)
+{
+    const double y = __builtin_amdgcn_rsq(x);
+    double g = x * y;
+    double h = y * 0.5;
+    const double r = __builtin_fma(-h, g, 0.5);
+    g = __builtin_fma(g, r, g);
+    double d = __builtin_fma(-g, g, x);
+    h = __builtin_fma(h, r, h);
+    g = __builtin_fma(d, h, g);
+    d = __builtin_fma(-g, g, x);
+    g = __builtin_fma(d, h, g);
+    return __builtin_amdgcn_class(x, 0x260) ? x : g;
+}
+
 __device__ __forceinline__ int comp(const int4 &q, int i)
 {
     return i == 0 ? q.x : (i == 1 ? q.y : (i == 2 ? q.z : q.w));
@@ -270,7 +290,7 @@ __device__ void cell_ref_body(const int v, const BlockIn &in, const GridP &gp,
                         const float px = aby * acz - abz * acy;
                         const float py = abz * acx - abx * acz;
                         const float pz = abx * acy - aby * acx;
-                        area = (float)((double)area + sqrt((double)nrm3(px, py, pz)) / 2.0);
+                        area = (float)((double)area + sqrt_of_float((double)nrm3(px, py, pz)) / 2.0);
                     }
                     cx = c.x;
                     cy = c.y;

@@ -35,8 +35,11 @@ __device__ __forceinline__ void push_point(long long lidx, float m, unsigned key
     } else {
         const unsigned e = atomicAdd(&o.ctrs[C_OVF], 1u);
         if (e < o.ovf_cap) {
-            const int prev = atomicExch(&o.head[lidx], (int)e);
-            o.ovf[e] = make_int4((int)lidx, (int)key, __float_as_int(m), prev);
+            // head only has to name one list entry of the point (the fold
+            // takes that entry as the point's representative): a plain store,
+            // the last writer wins, no round trip
+            o.head[lidx] = (int)e;
+            o.ovf[e] = make_int4((int)lidx, (int)key, __float_as_int(m), 0);
         }
     }
 }

@@ -55,7 +55,7 @@ __global__ __launch_bounds__(256) void k_fold_gather(const CellOut *__restrict__
         unsigned cnt = 0;
         if (e < n) {
             lidx = o.ovf[e].x;
-            mine = o.head[lidx] == (int)e;   // the newest entry of its point
+            mine = o.head[lidx] == (int)e;   // the entry that represents its point
             if (mine) {
                 cnt = o.count[lidx];
                 cls = cnt <= FOLD_SMALL_MAX ? 0 : (cnt <= FOLD_MID_MAX ? 1 : 2);

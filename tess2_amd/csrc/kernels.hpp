@@ -62,8 +62,8 @@ struct CellOut {
     float *density;    // [grid]
     unsigned *count;   // [grid] contributions per point
     uint2 *first;      // [grid] first contribution (cell, mass bits)
-    int *head;         // [grid] newest shared-list entry of the point
-    int4 *ovf;         // shared-point list: {lidx, cell, mass bits, next}
+    int *head;         // [grid] one shared-list entry of the point (its representative)
+    int4 *ovf;         // shared-point list: {lidx, key, mass bits, 0}
     unsigned ovf_cap;
     unsigned *ctrs;    // [C_NCTR]
     int *slow;         // cells for the large-capacity pass

@@ -62,6 +62,8 @@ __device__ __forceinline__ double sqrt_of_float(double x)
     return __builtin_amdgcn_class(x, 0x260) ? x : g;
 }
 
+__device__ __forceinline__ bool wide_bit(unsigned long long w, int b) { return (w >> b) & 1ull; }
+
 __device__ __forceinline__ int comp(const int4 &q, int i)
 {
     return i == 0 ? q.x : (i == 1 ? q.y : (i == 2 ? q.z : q.w));
@@ -77,7 +79,7 @@ __device__ __forceinline__ int comp(const int4 &q, int i)
 template <int NS> constexpr int hash_entries() { return 2 * 64 * NS; }
 template <int NS> constexpr int hash_ints() { return 2 * hash_entries<NS>() + 2 * 64 * NS; }
 
-template <int NS, int NF, int RMAX>
+template <int NS, int NF, int RMAX, bool WIDE = false>
 __device__ void cell_ref_body(const int v, const BlockIn &in, const GridP &gp,
                               const CellOut &o, const int want_density, const bool can_defer,
                               int4 *s_v, int *s_ns, float4 *s_cc, int4 *s_u, int *s_h,
@@ -426,10 +428,18 @@ __device__ void cell_ref_body(const int v, const BlockIn &in, const GridP &gp,
         const int lb2 = n2 > 1 ? 32 - __clz(n2 - 1) : 0;
         const int lb01 = lb0 + lb1;
         const bool use_mask = lb0 <= 5 && lb01 + lb2 <= 6;
+        // Wider boxes (up to 512 points, lines of at most 32) in the layout
+        // x + n0 (y + n1 z), 64 points per mask word (the batched kernel only:
+        // its sparse blocks have such boxes, the dense single-block inputs
+        // hardly any)
+        const int nbox = n0 * n1 * n2;
+        const bool use_wide = WIDE && !use_mask && n0 <= 32 && nbox <= 512;
         unsigned long long mk = 0ull;
-        if (use_mask) {
-            float4 *s_fa = s_cc;   // star records and circumcenters are done
-            float2 *s_fb = reinterpret_cast<float2 *>(s_v);
+        unsigned long long w1 = 0ull, w2 = 0ull, w3 = 0ull, w4 = 0ull, w5 = 0ull, w6 = 0ull,
+                           w7 = 0ull;
+        float4 *s_fa = s_cc;   // star records and circumcenters are done
+        float2 *s_fb = reinterpret_cast<float2 *>(s_v);
+        if (use_mask || use_wide) {
             __syncthreads();
 #pragma unroll
             for (int j = 0; j < NF; ++j) {
@@ -440,6 +450,45 @@ __device__ void cell_ref_body(const int v, const BlockIn &in, const GridP &gp,
                 }
             }
             __syncthreads();
+        }
+        if (use_wide) {
+            // point p = x + n0 (y + n1 z): quotients by the float reciprocal
+            // of the divisor, exact for p < 512 (the error stays far below
+            // the 0.5 / n margin of p + 0.5)
+            const int n01 = n0 * n1;
+            const float r01 = 1.0f / (float)n01, r0 = 1.0f / (float)n0;
+            const int nw = (nbox + 63) >> 6;
+            for (int c = 0; c < nw; ++c) {
+                const int p = c * 64 + lane;
+                const int z = (int)(((float)p + 0.5f) * r01);
+                const int rem = p - z * n01;
+                const int y = (int)(((float)rem + 0.5f) * r0);
+                const int x = rem - y * n0;
+                const float qx = p0 + (float)x * gp.step[0];
+                const float qy = p1 + (float)y * gp.step[1];
+                const float qz = p2 + (float)z * gp.step[2];
+                float hi = -INFINITY, lo = INFINITY;
+#pragma unroll 4
+                for (int k = 0; k < F; ++k) {
+                    const float4 a = s_fa[k];
+                    const float2 b = s_fb[k];
+                    const float d = a.x * (qx - a.w) + a.y * (qy - b.x) + a.z * (qz - b.y);
+                    const bool sig = fabsf(d) > eps;
+                    hi = fmaxf(hi, sig ? d : -INFINITY);
+                    lo = fminf(lo, sig ? d : INFINITY);
+                }
+                const unsigned long long m = __ballot(p < nbox && !(hi >= 0.0f && lo < 0.0f));
+                if (c == 0) mk = m;
+                else if (c == 1) w1 = m;
+                else if (c == 2) w2 = m;
+                else if (c == 3) w3 = m;
+                else if (c == 4) w4 = m;
+                else if (c == 5) w5 = m;
+                else if (c == 6) w6 = m;
+                else w7 = m;
+            }
+        }
+        if (use_mask) {
             const int x = lane & ((1 << lb0) - 1), y = (lane >> lb0) & ((1 << lb1) - 1);
             const int z = lane >> lb01;
             const float qx = p0 + (float)x * gp.step[0];
@@ -478,8 +527,13 @@ __device__ void cell_ref_body(const int v, const BlockIn &in, const GridP &gp,
             return !(__ballot(pos) != 0ull && __ballot(neg) != 0ull);
         };
 
+#define WORD(k)                                                                               \
+    ((k) == 0 ? mk : (k) == 1 ? w1 : (k) == 2 ? w2 : (k) == 3 ? w3 : (k) == 4 ? w4 : (k) == 5 ? w5 \
+     : (k) == 6 ? w6 : w7)
 #define PROBE(xi, yi, zi)                                                                     \
     (use_mask ? (bool)((mk >> ((xi) | ((yi) << lb0) | ((zi) << lb01))) & 1ull)                 \
+     : use_wide ? wide_bit(WORD(((xi) + n0 * ((yi) + n1 * (zi))) >> 6),                        \
+                           ((xi) + n0 * ((yi) + n1 * (zi))) & 63)                               \
               : inside(p0 + (float)(xi) * gp.step[0], p1 + (float)(yi) * gp.step[1],             \
                        p2 + (float)(zi) * gp.step[2]))
 
@@ -504,6 +558,13 @@ __device__ void cell_ref_body(const int v, const BlockIn &in, const GridP &gp,
                     if (use_mask) {
                         const int base = (yi << lb0) | (zi << lb01);
                         const unsigned long long row = (mk >> base) & ((1ull << n0) - 1ull);
+                        walk_row_bits(row, n0, x_left, x_right, min_xi, max_xi);
+                    } else if (use_wide) {
+                        // the line's n0 <= 32 bits, from one word or two
+                        const int base = n0 * (yi + n1 * zi), k = base >> 6, off = base & 63;
+                        unsigned long long row = WORD(k) >> off;
+                        if (off + n0 > 64) row |= WORD(k + 1) << (64 - off);
+                        row &= (1ull << n0) - 1ull;
                         walk_row_bits(row, n0, x_left, x_right, min_xi, max_xi);
                     } else {
                     // the left walk starts where x_in was taken: its first
@@ -586,6 +647,7 @@ __device__ void cell_ref_body(const int v, const BlockIn &in, const GridP &gp,
             }
         }
 #undef PROBE
+#undef WORD
         __syncthreads();
 
         // ---- 6. deposit -------------------------------------------------------
@@ -719,7 +781,7 @@ __global__ __launch_bounds__(64, 6) void k_cells_ref_batched(const BlockIn *__re
         for (int j = 0; j < ch.count; ++j) {
             const int k = ch.start + j;
             const int v = in.order ? in.order[k] : k;
-            cell_ref_body<NS, NF, RMAX>(v, in, gp, o, want_density, true, s_v, s_ns, s_cc, s_u,
+            cell_ref_body<NS, NF, RMAX, true>(v, in, gp, o, want_density, true, s_v, s_ns, s_cc, s_u,
                                         s_h, rt, ch.block);
             __syncthreads();
         }

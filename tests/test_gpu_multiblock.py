@@ -36,6 +36,9 @@ def _uniform_blocks(nside, per_block, seed, ghost):
 def multiblocks():
     out = {}
     out["uniform8"] = _uniform_blocks(2, 1200, 51, 0.1)
+    # few particles on a fine grid: cell boxes of ~100-500 points (the batched
+    # kernel's multi-word verdict masks) and beyond (point-by-point probes)
+    out["sparse8"] = _uniform_blocks(2, 150, 53, 0.2)
     pts, bid, counts, bounds = synthetic.plummer_blocks(2, 500, 4000, seed=52)
     out["plummer8"] = ingest.decomposed_blocks(pts, bid, bounds, 0.05)
     return out
@@ -48,7 +51,8 @@ def _bits(a):
     return u
 
 
-@pytest.mark.parametrize("name,g", [("uniform8", 33), ("uniform8", 40), ("plummer8", 33)])
+@pytest.mark.parametrize("name,g", [("uniform8", 33), ("uniform8", 40), ("plummer8", 33),
+                                    ("sparse8", 48), ("sparse8", 72)])
 def test_multiblock_reference_scan_bitexact(oracle, multiblocks, name, g):
     blocks = multiblocks[name]
     say("multiblock", name, g, [b.num_orig for b in blocks])

@@ -103,20 +103,35 @@ void launch_fill_u32(unsigned *p, size_t n, unsigned v, hipStream_t s)
 
 // per block (blockIdx.y): status histogram (5 codes) and total points of
 // deposited cells
-__global__ void k_cell_stats(const CellOut *__restrict__ outs)
+__global__ __launch_bounds__(256) void k_cell_stats(const CellOut *__restrict__ outs)
 {
+    // cells per status counted by ballots (wave-uniform counters), points of
+    // deposited cells summed per lane; one atomic per counter per workgroup
+    __shared__ unsigned long long s_c[4][6];
     const CellOut &o = outs[blockIdx.y];
-    unsigned long long c[6] = {0, 0, 0, 0, 0, 0};
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < o.ncells;
-         i += gridDim.x * blockDim.x) {
-        const int s = o.status[i];
-        if (s >= 0 && s < 5) c[s]++;
-        if (s == 0 || s == 1) c[5] += (unsigned long long)o.npts[i];
+    unsigned long long c[5] = {0, 0, 0, 0, 0}, np = 0;
+    for (int base = blockIdx.x * blockDim.x; base < o.ncells; base += gridDim.x * blockDim.x) {
+        const int i = base + threadIdx.x;
+        int s = -1;
+        if (i < o.ncells) {
+            s = o.status[i];
+            if (s == 0 || s == 1) np += (unsigned long long)o.npts[i];
+        }
+#pragma unroll
+        for (int k = 0; k < 5; ++k) c[k] += (unsigned long long)__popcll(__ballot(s == k));
     }
-    for (int k = 0; k < 6; ++k) {
-        unsigned long long x = c[k];
-        for (int off = 32; off >= 1; off >>= 1) x += __shfl_xor(x, off);
-        if ((threadIdx.x & 63) == 0 && x) atomicAdd(&o.st->cstat[k], x);
+    for (int off = 32; off >= 1; off >>= 1) np += __shfl_xor(np, off);
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+#pragma unroll
+        for (int k = 0; k < 5; ++k) s_c[w][k] = c[k];
+        s_c[w][5] = np;
+    }
+    __syncthreads();
+    if (threadIdx.x < 6) {
+        const int k = threadIdx.x;
+        const unsigned long long x = s_c[0][k] + s_c[1][k] + s_c[2][k] + s_c[3][k];
+        if (x) atomicAdd(&o.st->cstat[k], x);
     }
 }
 
@@ -130,14 +145,29 @@ __global__ __launch_bounds__(256) void k_grid_stats(const CellOut *__restrict__ 
 {
     __shared__ double s_acc[4];
     const CellOut &o = outs[blockIdx.y];
-    double acc = 0.0;
+    // 16-byte loads, four independent partial sums per lane (the grid
+    // arrays are 256-byte aligned allocations); the last n mod 4 points
+    // go to the first lanes of workgroup 0
+    const unsigned long long n = o.ngrid, n4 = n >> 2;
+    const float4 *d4 = reinterpret_cast<const float4 *>(o.density);
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
     float mx = 0.0f;
-    for (unsigned long long i = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;
-         i < o.ngrid; i += (unsigned long long)gridDim.x * blockDim.x) {
-        const float x = o.density[i];
-        acc += (double)x;
+#pragma unroll 4
+    for (unsigned long long i = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x; i < n4;
+         i += (unsigned long long)gridDim.x * blockDim.x) {
+        const float4 x = d4[i];
+        a0 += (double)x.x;
+        a1 += (double)x.y;
+        a2 += (double)x.z;
+        a3 += (double)x.w;
+        mx = fmaxf(mx, fmaxf(fmaxf(x.x, x.y), fmaxf(x.z, x.w)));
+    }
+    if (blockIdx.x == 0 && threadIdx.x < (unsigned)(n & 3ull)) {
+        const float x = o.density[(n4 << 2) + threadIdx.x];
+        a0 += (double)x;
         mx = fmaxf(mx, x);
     }
+    double acc = (a0 + a1) + (a2 + a3);
     for (int off = 32; off >= 1; off >>= 1) {
         acc += __shfl_xor(acc, off);
         mx = fmaxf(mx, __shfl_xor(mx, off));

@@ -139,6 +139,18 @@ __device__ __forceinline__ void cas_pair(unsigned *kk, unsigned *mm, int j)
     mm[j + 1] = m1;
 }
 
+// odd-even transposition sort of the first P register entries (P rounds)
+template <int P> __device__ __forceinline__ void oe_sort(unsigned *kk, unsigned *mm)
+{
+#pragma unroll
+    for (int rd = 0; rd < P / 2; ++rd) {
+#pragma unroll
+        for (int j = 0; j + 1 < P; j += 2) cas_pair(kk, mm, j);
+#pragma unroll
+        for (int j = 1; j + 1 < P; j += 2) cas_pair(kk, mm, j);
+    }
+}
+
 // 3a. at most 16 contributions: one lane per point, odd-even transposition
 // sort in registers
 __global__ void k_fold_small(const CellOut *__restrict__ outs, const GridP *__restrict__ gps)
@@ -165,13 +177,13 @@ __global__ void k_fold_small(const CellOut *__restrict__ outs, const GridP *__re
                 mm[j] = q.y;
             }
         }
-#pragma unroll
-        for (int rd = 0; rd < FOLD_SMALL_MAX / 2; ++rd) {
-#pragma unroll
-            for (int j = 0; j + 1 < FOLD_SMALL_MAX; j += 2) cas_pair(kk, mm, j);
-#pragma unroll
-            for (int j = 1; j + 1 < FOLD_SMALL_MAX; j += 2) cas_pair(kk, mm, j);
-        }
+        // the network spans the wave's largest count rounded up to 2, 4, 8
+        // or 16 (most shared points have two contributions); the unused
+        // entries hold the largest key and stay at the end
+        if (__ballot(cnt > 8)) oe_sort<16>(kk, mm);
+        else if (__ballot(cnt > 4)) oe_sort<8>(kk, mm);
+        else if (__ballot(cnt > 2)) oe_sort<4>(kk, mm);
+        else oe_sort<2>(kk, mm);
         // sorted back into the segment, then one rolled chain (no dynamic
         // register indexing)
 #pragma unroll

@@ -804,8 +804,8 @@ __global__ __launch_bounds__(64) void k_cells_ref_slow_batched(const BlockIn *__
         unsigned n = o.ctrs[C_SLOW];
         if (n > o.slow_cap) n = o.slow_cap;
         for (unsigned k = blockIdx.x; k < n; k += gridDim.x) {
-            cell_ref_body<NS, NF, RMAX>(o.slow[k], ins[b], rt->gps[b], o, want_density, false, s_v,
-                                        s_ns, s_cc, s_u, s_h, rt, b);
+            cell_ref_body<NS, NF, RMAX, true>(o.slow[k], ins[b], rt->gps[b], o, want_density,
+                                              false, s_v, s_ns, s_cc, s_u, s_h, rt, b);
             __syncthreads();
         }
     }

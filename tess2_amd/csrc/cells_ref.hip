@@ -10,16 +10,19 @@
 //   2. faces: the Delaunay neighbours u in first-seen order with their first
 //      tet ut (one face per lane); each lane walks its edge ring in LDS
 //      (fill_edge_link, src/tet.cpp:389-409) and accumulates, in ring order,
-//      the Newell normal (src/dense.cpp:1131-1162), the fan area
-//      (src/volume.cpp:32-46) and the ring's bounding box.
+//      the Newell normal (src/dense.cpp:1131-1162) and the fan area
+//      (src/volume.cpp:32-46).
 //   3. volume = sum of area*|u-v|/6 over faces in neighbour order
 //      (src/volume.cpp:23-53).
-//   4. data-bounds filter and bbox -> grid range (src/dense.cpp:1381-1410).
+//   4. cell bbox over the star's circumcenters (the ring vertices), data-
+//      bounds filter, bbox -> grid range (src/dense.cpp:701-714,1381-1410).
 //   5. scanline search (CellInteriorGridPts, src/dense.cpp:1475-1701) as a
-//      wave-uniform state machine; every PtInCell probe evaluates all faces
-//      at once, one per lane, and two ballots give the verdict
-//      (src/dense.cpp:1172-1203: inside iff no two significant faces
-//      disagree in sign).
+//      wave-uniform state machine.  PtInCell (src/dense.cpp:1172-1203: inside
+//      iff no two significant faces disagree in sign) is evaluated for the
+//      whole cell box at once when it is small, one lane per point, into
+//      bit masks whose lines the walks scan with bit operations; larger
+//      boxes probe point by point, all faces at once (one per lane) with
+//      two ballots per verdict.
 //   6. deposit: mass/N per found point (or CIC of the site when N = 0).
 //      The first contribution to a grid point writes the density directly;
 //      later ones go to a shared-point list that an ordered pass folds in

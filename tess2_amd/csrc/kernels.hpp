@@ -120,27 +120,18 @@ struct BlockIn {
 };
 
 // find or insert key (>= 0) in an open-addressing table of HS entries in
-// LDS (-1 = empty); returns the entry, or -1 when HS probes found no room.
-// inserted: this lane's CAS created the entry (one lane per new key).
+// LDS (-1 = empty); returns the entry, or -1 when HS probes found no room
 template <int HS>
-__device__ __forceinline__ int hash_find(int *hk, int key, bool &inserted)
+__device__ __forceinline__ int hash_find(int *hk, int key)
 {
     unsigned h = ((unsigned)key * 0x9E3779B1u) >> 16;
-    inserted = false;
     for (int n = 0; n < HS; ++n) {
         h &= HS - 1;
         const int old = atomicCAS(&hk[h], -1, key);
-        if (old == -1) inserted = true;
         if (old == -1 || old == key) return (int)h;
         ++h;
     }
     return -1;
-}
-template <int HS>
-__device__ __forceinline__ int hash_find(int *hk, int key)
-{
-    bool ins;
-    return hash_find<HS>(hk, key, ins);
 }
 
 // Workgroup -> work item with XCD affinity: the dispatcher hands consecutive

@@ -101,6 +101,7 @@ def cpu_baseline_blocks(blocks, grid, target_s):
     a = O.make_args(1.0, dmin, dmax, gmin, step, eps, grid)
     order = np.random.default_rng(0).permutation(len(blocks))
     spent, cells, used = 0.0, 0, 0
+    sampled = []   # (gid, status, npts) of every sampled block
     for i in order:
         hb = blocks[i]
         ob = O.Block.from_host(hb)
@@ -108,10 +109,11 @@ def cpu_baseline_blocks(blocks, grid, target_s):
                                          np.asarray(hb.bounds_max, np.float32), gmin, step,
                                          eps, dmin, dmax, grid)
         t0 = time.perf_counter()
-        O.circumcenters(hb.tets, hb.particles)
-        O.volumes(ob)
-        O.iterate_cells(ob, a, mn, nm)
+        cc = O.circumcenters(hb.tets, hb.particles)
+        O.volumes(ob, cc)
+        _, status, npts, _, _ = O.iterate_cells(ob, a, mn, nm)
         spent += time.perf_counter() - t0
+        sampled.append((hb.gid, status, npts))
         cells += hb.num_orig
         used += 1
         if spent >= target_s:
@@ -121,7 +123,8 @@ def cpu_baseline_blocks(blocks, grid, target_s):
     return {"est_full_s": est, "cores": 1, "kind": "port",
             "sample": f"{used} of {len(blocks)} whole blocks ({cells} of {total} cells, "
                       f"{spent:.1f}s of CPU work: circumcenters, volume() and IterateCells "
-                      f"per block, own deposits only), scaled by cell count; single thread"}
+                      f"per block, own deposits only), scaled by cell count; single thread",
+            "sampled": sampled}
 
 
 def cpu_baseline(hb, grid, target_s):
@@ -139,11 +142,14 @@ def cpu_baseline(hb, grid, target_s):
         t0 = time.perf_counter()
         cc = O.circumcenters(hb.tets[:mt], hb.particles)
         t1 = time.perf_counter()
-        O.volumes(sub)
-        O.dense_single(sub, grid)
+        vol = O.volumes(sub, cc_all)
+        d = O.dense_single(sub, grid)
         t2 = time.perf_counter()
+        timed.result = (vol, d["status"], d["npts"])
         return (t1 - t0) * T / mt + (t2 - t1) * n / m, t2 - t0
 
+    cc_all = O.circumcenters(hb.tets, hb.particles)   # for volume(), untimed: the
+    # circumcenter pass is timed on its own fraction of the tets below
     m = min(n, 20000)
     est, spent = timed(m)
     per_cell = spent / m
@@ -157,7 +163,7 @@ def cpu_baseline(hb, grid, target_s):
             "sample": f"first {m} of {n} cells ({spent:.1f}s of CPU work: circumcenters of "
                       f"the same fraction of tets, volume() and IterateCells over those "
                       f"cells), scaled to all cells; single thread",
-            "est_full_s": est}
+            "est_full_s": est, "sampled": timed.result}
 
 
 def exact_paths(run, grid, G, args):
@@ -194,6 +200,33 @@ def exact_paths(run, grid, G, args):
     res["max_rel_density_diff"] = float(rel.max()) if rel.size else 0.0
     res["rms_rel_density_diff"] = float(np.sqrt(np.mean(rel ** 2))) if rel.size else 0.0
     return res
+
+
+def _vol_bits(a):
+    a = np.ascontiguousarray(a, np.float32)
+    u = a.view(np.uint32).copy()
+    u[np.isnan(a)] = 0x7FC00000
+    return u
+
+
+def sample_parity(sampled, gpu_cells, gpu_vol, multi):
+    """The GPU's per-cell statuses and point counts (and volumes, one block)
+    of the full-size run against the oracle's on the cells the CPU baseline
+    processed."""
+    if multi:
+        ok, cells = True, 0
+        for gid, status, npts in sampled:
+            gs, gn = gpu_cells[gid]
+            ok = ok and np.array_equal(gs, status) and np.array_equal(gn, npts)
+            cells += len(status)
+        return {"cells": cells, "status_npts_identical": bool(ok)}
+    vol, status, npts = sampled
+    m = len(status)
+    gs, gn = next(iter(gpu_cells.values()))
+    return {"cells": m,
+            "status_npts_identical": bool(np.array_equal(gs[:m], status) and
+                                          np.array_equal(gn[:m], npts)),
+            "volumes_identical": bool(np.array_equal(_vol_bits(gpu_vol[:m]), _vol_bits(vol)))}
 
 
 def main():
@@ -311,6 +344,10 @@ def main():
                      "kernel": kernel, "kernel_ms": kern_ms, "alg_bytes": b_alg},
     }
     run.set_option("graph", 0)
+    # per-cell results of the reference scan, for the check against the
+    # oracle's sample below (the exact runs overwrite them)
+    gpu_cells = {b.gid: run.cells(i) for i, b in enumerate(run.blocks)}
+    gpu_vol = run.fetch(0, ngrid=False, volumes=True)["volumes"] if not multi else None
     if not args.no_exact and not multi:
         out["exact_voronoi"] = exact_paths(run, grid, G, args)
     if not args.no_cpu:
@@ -321,6 +358,8 @@ def main():
             cb = cpu_baseline(blocks[0], grid, args.cpu_seconds)
         cb["value"] = G / cb["est_full_s"]
         cb["unit"] = "grid pts/s"
+        out["sample_parity"] = sample_parity(cb.pop("sampled"), gpu_cells, gpu_vol, multi)
+        log(f"[cpu] sample parity: {out['sample_parity']}")
         out["cpu_baseline"] = cb
     print(json.dumps(out), flush=True)
 

@@ -166,10 +166,12 @@ def cpu_baseline(hb, grid, target_s):
             "est_full_s": est, "sampled": timed.result}
 
 
-def exact_paths(run, grid, G, args):
+def exact_paths(run, grid, G, args, ref_density):
     """The exact-Voronoi assignment in the mixed FP32/FP64 path and in the
-    all-FP64 path on the same staged block: timing, FP64 fallbacks, and the
-    agreement of the two (owner of every grid point, densities)."""
+    all-FP64 path on the same staged block: timing, FP64 fallbacks, the
+    agreement of the two (owner of every grid point, densities), and how the
+    reference scan's point sets differ from the exact assignment (grid points
+    the scan leaves without mass that the exact assignment gives to a cell)."""
     from tess2_amd import host
     res = {}
     outs = {}
@@ -199,6 +201,11 @@ def exact_paths(run, grid, G, args):
     res["owner_identical"] = bool(np.array_equal(om, of))
     res["max_rel_density_diff"] = float(rel.max()) if rel.size else 0.0
     res["rms_rel_density_diff"] = float(np.sqrt(np.mean(rel ** 2))) if rel.size else 0.0
+    scan0, ex0 = ref_density == 0, dm == 0
+    res["scan_vs_exact"] = {"points_massless_scan": int(scan0.sum()),
+                            "points_massless_exact": int(ex0.sum()),
+                            "points_missed_by_scan": int((scan0 & ~ex0).sum()),
+                            "grid_points": int(G)}
     return res
 
 
@@ -349,7 +356,8 @@ def main():
     gpu_cells = {b.gid: run.cells(i) for i, b in enumerate(run.blocks)}
     gpu_vol = run.fetch(0, ngrid=False, volumes=True)["volumes"] if not multi else None
     if not args.no_exact and not multi:
-        out["exact_voronoi"] = exact_paths(run, grid, G, args)
+        ref_density = run.fetch(0)["density"]
+        out["exact_voronoi"] = exact_paths(run, grid, G, args, ref_density)
     if not args.no_cpu:
         log("[cpu] timing the oracle on a sample")
         if multi:

@@ -49,6 +49,13 @@ typedef struct tess2_block_t {
     const int *vert_to_tet;          /* num_particles entries, -1 = none */
     float bounds_min[3];
     float bounds_max[3];
+    /* optional (NULL when absent), for the exact assignment over several
+       blocks: for each ghost (particles num_orig_particles..num_particles-1)
+       the gid of the block that holds it as an original, and its index
+       there.  No reference counterpart: the reference's ghost exchange
+       (src/tess.cpp:492-628) knows this but does not keep it. */
+    const int *ghost_gid;
+    const int *ghost_cell;
 } tess2_block_t;
 
 /* estimator algorithm, enum alg in include/tess/dense.hpp:33-38 */

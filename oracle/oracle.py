@@ -330,6 +330,80 @@ def exact_dense(block, gnum, mass=1.0, eps=1e-4):
                 grid_phys_mins=gmin, block_num_idx=bnum, args=args)
 
 
+def exact_dense_multi(blocks, keys, gnum, mass=1.0):
+    """Exact-Voronoi density over several blocks (FP64 oracle).  blocks: in
+    rank (ascending gid) order; keys[k]: home key (rank << 23 | cell) of each
+    particle of block k, originals and ghosts.  Every in-block grid point of
+    block k goes to the nearest of block k's particles (ties: lowest home
+    key); a home cell's point count is summed over all blocks, and each
+    point holds mass / count of its owner when that cell deposits.  Cells
+    that deposit but own no point are counted in "cic" and their status and
+    count set as the single-block assignment does (CIC, 8); their CIC
+    deposits are not modelled here.  Returns a dict of per-block lists:
+    owner (home keys, -1 outside the block), status, npts, density (float64)."""
+    dmin = np.min([np.array(b.cs.bounds_min[:], np.float32) for b in blocks], axis=0)
+    dmax = np.max([np.array(b.cs.bounds_max[:], np.float32) for b in blocks], axis=0)
+    gmin, gmax, step = grid_step_params(dmin, dmax, gnum)
+    eps = 1e-4
+    args = make_args(mass, dmin, dmax, gmin, step, eps, gnum)
+    owners, stats = [], []
+    for b, key in zip(blocks, keys):
+        mn, mx, nm = block_grid_params(np.array(b.cs.bounds_min[:], np.float32),
+                                       np.array(b.cs.bounds_max[:], np.float32), gmin, step,
+                                       eps, dmin, dmax, gnum)
+        ng = int(nm[0]) * int(nm[1]) * int(nm[2])
+        own = np.zeros(ng, np.int32)
+        st = np.zeros(b.num_orig, np.int32)
+        npt = np.zeros(b.num_orig, np.int32)
+        den = np.zeros(ng, np.float64)
+        lib().o_exact_dense(C.byref(b.cs), C.byref(args), mn, nm, own, st, npt, den)
+        # statuses before the CIC step: cells that deposit are O_CELL_SCAN
+        st[st == 1] = 0
+        key = np.asarray(key, np.int64)
+        perm = np.argsort(key, kind="stable")
+        p = np.ascontiguousarray(b.particles.reshape(-1, 3)[perm]).reshape(-1)
+        o = np.zeros(ng, np.int32)
+        lib().o_exact_owner_off(p, p.size // 3, mn, nm, step, gmin, o)
+        owners.append(np.where(own < 0, -1, key[perm][o]).astype(np.int64))
+        stats.append(st)
+    # every home cell at a flat position: base[rank] + cell
+    base = np.concatenate([[0], np.cumsum([b.num_orig for b in blocks])]).astype(np.int64)
+    dep_all = np.concatenate(stats) == 0
+    norig = np.array([b.num_orig for b in blocks], np.int64)
+    cnt = np.zeros(int(base[-1]), np.int64)
+
+    def homed(ok):
+        # a key naming no original (a ghost id of a single block) deposits nothing
+        return (ok >= 0) & ((ok & 0x7FFFFF) < norig[np.maximum(ok, 0) >> 23])
+
+    for ok in owners:
+        ok = ok[homed(ok)]
+        f = base[ok >> 23] + (ok & 0x7FFFFF)
+        np.add.at(cnt, f[dep_all[f]], 1)
+    npts = [cnt[base[k]:base[k + 1]] for k in range(len(blocks))]
+    div = float(np.float32(step[0] * step[1] * step[2]))
+    dens = []
+    for ok in owners:
+        d = np.zeros(ok.size, np.float64)
+        g = np.nonzero(homed(ok))[0]
+        f = base[ok[g] >> 23] + (ok[g] & 0x7FFFFF)
+        keep = dep_all[f]
+        d[g[keep]] = mass / cnt[f[keep]] / div
+        dens.append(d)
+    cic = 0
+    status, counts = [], []
+    for st, n in zip(stats, npts):
+        st = st.copy()
+        n = np.where(st == 0, n, 0)
+        z = (st == 0) & (n == 0)
+        cic += int(z.sum())
+        st[z] = 1
+        n[z] = 8
+        status.append(st)
+        counts.append(n.astype(np.int32))
+    return dict(owner=owners, status=status, npts=counts, density=dens, cic=cic)
+
+
 def dense_multi(blocks, gnum, mass=1.0, eps=1e-4, cic=False):
     """dense() over several blocks (src/dense.cpp:30-103): each block deposits
     its own cells' points that fall in its bounds and grid (IterateCells, in

@@ -31,7 +31,8 @@ class Block(C.Structure):
                 ("num_particles", C.c_int), ("particles", C.c_void_p),
                 ("num_tets", C.c_int), ("tets", C.c_void_p),
                 ("vert_to_tet", C.c_void_p), ("bounds_min", C.c_float * 3),
-                ("bounds_max", C.c_float * 3)]
+                ("bounds_max", C.c_float * 3), ("ghost_gid", C.c_void_p),
+                ("ghost_cell", C.c_void_p)]
 
 
 class Stats(C.Structure):
@@ -171,4 +172,14 @@ def make_block(hb) -> tuple[Block, list]:
     b.vert_to_tet = v.ctypes.data
     b.bounds_min[:] = [float(x) for x in np.asarray(hb.bounds_min, np.float32)]
     b.bounds_max[:] = [float(x) for x in np.asarray(hb.bounds_max, np.float32)]
-    return b, [p, t, v]
+    keep = [p, t, v]
+    gg, gc = getattr(hb, "ghost_gid", None), getattr(hb, "ghost_cell", None)
+    if gg is not None and gc is not None:
+        gg = np.ascontiguousarray(gg, dtype=np.int32)
+        gc = np.ascontiguousarray(gc, dtype=np.int32)
+        if gg.shape != (p.shape[0] - int(hb.num_orig),) or gc.shape != gg.shape:
+            raise ValueError("ghost_gid / ghost_cell need one entry per ghost")
+        b.ghost_gid = gg.ctypes.data
+        b.ghost_cell = gc.ctypes.data
+        keep += [gg, gc]
+    return b, keep

@@ -53,6 +53,8 @@ struct DevBlock {
     unsigned *hstart = nullptr, *hfill = nullptr, *hsums = nullptr;
     int *hbk = nullptr;
     float4 *hsorted = nullptr;
+    int *hkey = nullptr;                     // [np] home key of each particle (several blocks)
+    std::vector<int> ghost_gid, ghost_cell;  // home of each ghost, when given
     size_t owner_cap = 0;
     BlockStats *st = nullptr;                // this block's record in the context's array
     unsigned *ctrs = nullptr;                // st->ctrs
@@ -107,6 +109,12 @@ struct tess2_ctx {
     bool in_capture = false;
     BlockStats *d_stats = nullptr, *h_stats = nullptr;   // per-block counters, device / pinned
     double *d_gpart = nullptr;                           // partial grid sums, 1024 per block
+    // exact assignment over several blocks: every particle's home key
+    // (rank << 23 | cell), and the per-rank npts / deposit-status tables
+    bool homes_ok = false;
+    std::string homes_err;
+    int **d_npts_tab = nullptr;
+    int **d_stex_tab = nullptr;
     bool kevents = false;            // the last run recorded the kernel events
     RunArgs desc_args;
     unsigned long long desc_gen = 0;
@@ -127,7 +135,10 @@ static void drop_graph(tess2_ctx *c)
 static void free_multi(tess2_ctx *c)
 {
     void *ps[] = {c->d_ins, c->d_gps, c->d_outs, c->d_router, c->d_chunks, c->d_queue,
-                  c->d_stats, c->d_gpart};
+                  c->d_stats, c->d_gpart, c->d_npts_tab, c->d_stex_tab};
+    c->d_npts_tab = nullptr;
+    c->d_stex_tab = nullptr;
+    c->homes_ok = false;
     for (void *p : ps)
         if (p) (void)hipFree(p);
     if (c->h_stats) (void)hipHostFree(c->h_stats);
@@ -184,7 +195,7 @@ static void free_block(DevBlock &b)
     void *ps[] = {b.order,  b.pts,    b.tets,   b.v2t,   b.cc,   b.vol,  b.status,    b.npts,
                   b.density, b.count, b.first, b.head, b.ovf,  b.slow,
                   b.owner, b.status_ex, b.exflags, b.hstart, b.hfill,
-                  b.hsums,  b.hbk,    b.hsorted, b.seg, b.fold[0], b.fold[1], b.fold[2]};
+                  b.hsums,  b.hbk,    b.hsorted, b.hkey, b.seg, b.fold[0], b.fold[1], b.fold[2]};
     for (void *p : ps)
         if (p) (void)hipFree(p);
     b = DevBlock();
@@ -423,6 +434,65 @@ static int build_chunks(tess2_ctx *c)
     return 0;
 }
 
+// Home keys for the exact assignment over several blocks: particle p of the
+// block at rank k is (k << 23 | p) when it is an original, else its home
+// block's rank and cell.  Without a complete, consistent home for every ghost
+// the exact assignment over several blocks is refused at run time (the
+// reference scan does not need it).
+static int build_home_keys(tess2_ctx *c)
+{
+    const int nb = (int)c->blocks.size();
+    c->homes_ok = false;
+    std::vector<int> gids(nb);
+    for (int k = 0; k < nb; ++k) gids[k] = c->blocks[k].gid;
+    for (int k = 0; k < nb; ++k) {
+        const DevBlock &d = c->blocks[k];
+        if (d.np > d.norig && d.ghost_gid.empty()) {
+            c->homes_err = "block " + std::to_string(d.gid) + " gives no ghost homes";
+            return 0;
+        }
+    }
+    std::vector<int> key;
+    for (int k = 0; k < nb; ++k) {
+        DevBlock &d = c->blocks[k];
+        key.resize(d.np);
+        for (int p = 0; p < d.norig; ++p) key[p] = (k << 23) | p;
+        for (int p = d.norig; p < d.np; ++p) {
+            const int g = d.ghost_gid[p - d.norig], cell = d.ghost_cell[p - d.norig];
+            const auto it = std::lower_bound(gids.begin(), gids.end(), g);
+            if (it == gids.end() || *it != g) {
+                c->homes_err = "a ghost of block " + std::to_string(d.gid) + " names block " +
+                               std::to_string(g) + ", which is not staged";
+                return 0;
+            }
+            const int r = (int)(it - gids.begin());
+            if (cell < 0 || cell >= c->blocks[r].norig) {
+                c->homes_err = "a ghost of block " + std::to_string(d.gid) +
+                               " names a cell outside block " + std::to_string(g);
+                return 0;
+            }
+            key[p] = (r << 23) | cell;
+        }
+        int rr = 0;
+        if ((rr = dalloc(c, &d.hkey, (size_t)d.np))) return rr;
+        if (d.np)
+            HIPCHK(c, hipMemcpy(d.hkey, key.data(), sizeof(int) * (size_t)d.np,
+                                hipMemcpyHostToDevice));
+    }
+    std::vector<int *> tn(nb), ts(nb);
+    for (int k = 0; k < nb; ++k) {
+        tn[k] = c->blocks[k].npts;
+        ts[k] = c->blocks[k].status_ex;
+    }
+    int rr = 0;
+    if ((rr = dalloc(c, &c->d_npts_tab, (size_t)nb)) || (rr = dalloc(c, &c->d_stex_tab, (size_t)nb)))
+        return rr;
+    HIPCHK(c, hipMemcpy(c->d_npts_tab, tn.data(), sizeof(int *) * nb, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->d_stex_tab, ts.data(), sizeof(int *) * nb, hipMemcpyHostToDevice));
+    c->homes_ok = true;
+    return 0;
+}
+
 extern "C" int tess2_stage(tess2_ctx *c, const tess2_block_t *blocks, int nblocks)
 {
     if (!c || (nblocks > 0 && !blocks) || nblocks < 0) return fail(c, TESS2_E_ARG, "bad args");
@@ -460,6 +530,10 @@ extern "C" int tess2_stage(tess2_ctx *c, const tess2_block_t *blocks, int nblock
         d.norig = hb.num_orig_particles;
         d.np = hb.num_particles;
         d.nt = hb.num_tets;
+        if (hb.ghost_gid && hb.ghost_cell && d.np > d.norig) {
+            d.ghost_gid.assign(hb.ghost_gid, hb.ghost_gid + (d.np - d.norig));
+            d.ghost_cell.assign(hb.ghost_cell, hb.ghost_cell + (d.np - d.norig));
+        }
         for (int i = 0; i < 3; ++i) {
             d.bmin[i] = hb.bounds_min[i];
             d.bmax[i] = hb.bounds_max[i];
@@ -550,6 +624,7 @@ extern "C" int tess2_stage(tess2_ctx *c, const tess2_block_t *blocks, int nblock
         }
         r = build_chunks(c);
         if (r) return r;
+        if (nblocks > 1 && (r = build_home_keys(c))) return r;
     }
     return 0;
 }
@@ -638,8 +713,9 @@ static int prepare_run(tess2_ctx *c, const RunArgs &a)
     if (c->blocks.empty()) return fail(c, TESS2_E_ARG, "no staged blocks");
     const int nb = (int)c->blocks.size();
     const bool exact = a.alg == TESS2_DENSE_TESS && a.assignment == TESS2_ASSIGN_EXACT_VORONOI;
-    if (exact && nb > 1)
-        return fail(c, TESS2_E_ARG, "the exact assignment runs on one block at a time");
+    if (exact && nb > 1 && !c->homes_ok)
+        return fail(c, TESS2_E_ARG, "the exact assignment over several blocks: %s",
+                    c->homes_err.c_str());
     // DataBounds(): min/max over the block bounds (src/dense.cpp:1238-1267)
     for (int k = 0; k < nb; ++k)
         for (int i = 0; i < 3; ++i) {
@@ -730,23 +806,42 @@ static int enqueue_kernels(tess2_ctx *c, const RunArgs &a)
     }
     // phase B: the estimator
     if (exact) {
-        DevBlock &d = c->blocks[0];
-        const BlockIn &in = c->h_ins[0];
-        const GridP &gp = c->h_gps[0];
-        if (a.want_vol) {
-            // volume() as the reference computes it
-            CellOut ov = make_out(d, true);
-            launch_cells_ref(in, gp, ov, 0, s);
-            launch_cells_ref_slow(in, gp, ov, 0, s);
+        // several blocks: particle ids are home keys, every block's cell
+        // counts are complete before any density is written (all statuses,
+        // then all counts, then the densities), and off-block CIC corners
+        // are routed like the reference scan's points
+        const bool multi = nb > 1;
+        int *const *ntab = multi ? c->d_npts_tab : nullptr;
+        const int *const *stab = multi ? c->d_stex_tab : nullptr;
+        for (int k = 0; k < nb; ++k) {
+            DevBlock &d = c->blocks[k];
+            const BlockIn &in = c->h_ins[k];
+            const GridP &gp = c->h_gps[k];
+            if (a.want_vol) {
+                // volume() as the reference computes it
+                CellOut ov = make_out(d, true);
+                launch_cells_ref(in, gp, ov, 0, s);
+                launch_cells_ref_slow(in, gp, ov, 0, s);
+            }
+            launch_hash_build(d.pts, d.np, d.hash, d.hstart, d.hfill, d.hsums, d.hbk, d.hsorted,
+                              multi ? d.hkey : nullptr, s);
+            launch_exact_status(in, gp, d.status_ex, a.precision, d.ctrs,
+                                reinterpret_cast<unsigned char *>(d.exflags), s);
+            if (multi && d.norig > 0) launch_fill_u32((unsigned *)d.npts, (size_t)d.norig, 0u, s);
         }
-        launch_hash_build(d.pts, d.np, d.hash, d.hstart, d.hfill, d.hsums, d.hbk, d.hsorted, s);
-        launch_exact_status(in, gp, d.status_ex, a.precision, d.ctrs,
-                            reinterpret_cast<unsigned char *>(d.exflags), s);
         if (ev) HIPCHK(c, hipEventRecord(c->ev_k0, s));
-        launch_owner(gp, d.hash, d.hstart, d.hsorted, d.owner, a.precision, d.ctrs, d.status_ex,
-                     d.npts, d.norig, s);
+        for (int k = 0; k < nb; ++k) {
+            DevBlock &d = c->blocks[k];
+            launch_owner(c->h_gps[k], d.hash, d.hstart, d.hsorted, d.owner, a.precision, d.ctrs,
+                         d.status_ex, d.npts, d.norig, ntab, stab, s);
+        }
         if (ev) HIPCHK(c, hipEventRecord(c->ev_k1, s));
-        launch_exact_tail(in, gp, d.owner, d.status_ex, c->h_outs[0], a.precision, s);
+        // a cell's count changes in its CIC pass only when it owns no point,
+        // so no density pass reads a count another block's CIC pass rewrote
+        for (int k = 0; k < nb; ++k)
+            launch_exact_tail(c->h_ins[k], c->h_gps[k], c->blocks[k].owner,
+                              c->blocks[k].status_ex, c->h_outs[k], a.precision, ntab, stab,
+                              multi ? c->d_router : nullptr, k, s);
     } else if (a.alg == TESS2_DENSE_CIC) {
         if (ev) HIPCHK(c, hipEventRecord(c->ev_k0, s));
         for (int k = 0; k < nb; ++k)

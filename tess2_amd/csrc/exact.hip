@@ -106,18 +106,22 @@ __global__ void k_scan_add(unsigned *__restrict__ out, const unsigned *__restric
     }
 }
 
+// the fourth word is the particle's id: its index, or with several blocks
+// its home key (rank << 23 | cell), which also breaks distance ties
 __global__ void k_hash_scatter(const float4 *__restrict__ pts, int np, const int *__restrict__ bk,
-                               unsigned *__restrict__ fill, float4 *__restrict__ sorted)
+                               unsigned *__restrict__ fill, float4 *__restrict__ sorted,
+                               const int *__restrict__ key)
 {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= np) return;
     const unsigned slot = atomicAdd(&fill[bk[p]], 1u);
     const float4 q = pts[p];
-    sorted[slot] = make_float4(q.x, q.y, q.z, __int_as_float(p));
+    sorted[slot] = make_float4(q.x, q.y, q.z, __int_as_float(key ? key[p] : p));
 }
 
 void launch_hash_build(const float4 *pts, int np, const HashP &h, unsigned *start,
-                       unsigned *fill, unsigned *sums, int *bk, float4 *sorted, hipStream_t s)
+                       unsigned *fill, unsigned *sums, int *bk, float4 *sorted, const int *key,
+                       hipStream_t s)
 {
     const int nbk = h.dims[0] * h.dims[1] * h.dims[2];
     launch_fill_u32(start, (size_t)nbk + 1, 0u, s);
@@ -131,7 +135,7 @@ void launch_hash_build(const float4 *pts, int np, const HashP &h, unsigned *star
     hipLaunchKernelGGL(k_scan_add, dim3(nblk), dim3(1024), 0, s, start, sums, n, fill);
     if (np > 0)
         hipLaunchKernelGGL(k_hash_scatter, dim3((np + 255) / 256), dim3(256), 0, s, pts, np, bk,
-                           fill, sorted);
+                           fill, sorted, key);
 }
 
 // ---------------------------------------------------------------------------
@@ -276,12 +280,16 @@ __device__ int nearest_f32(float gx, float gy, float gz, const HashP &h,
 }
 
 // one thread per block grid point; a 64-lane wave covers a 4x4x4 tile
+// With several blocks the owner is a home key (rank << 23 | cell) and the
+// counts go to the home block's cell (npts_tab / stex_tab by rank).
 __global__ __launch_bounds__(256) void k_owner(GridP gp, HashP h,
                                                const unsigned *__restrict__ start,
                                                const float4 *__restrict__ hp, int *__restrict__ owner,
                                                int prec, unsigned *ctrs, int tiles_x, int tiles_y,
                                                int ntiles, const int *__restrict__ status,
-                                               int *__restrict__ npts, int ncells)
+                                               int *__restrict__ npts, int ncells,
+                                               int *const *__restrict__ npts_tab,
+                                               const int *const *__restrict__ stex_tab)
 {
     const int wave = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (wave >= ntiles) return;
@@ -311,28 +319,34 @@ __global__ __launch_bounds__(256) void k_owner(GridP gp, HashP h,
     }
     // points per deposited cell; the lanes of a tile mostly share owners, so
     // one atomic per distinct owner in the wave
-    const bool cnt = on_grid && in_block && o >= 0 && o < ncells && status[o] == EX_DEPOSIT;
+    const int hc = npts_tab ? (o & 0x7fffff) : o;
+    const int *hst = npts_tab && o >= 0 ? stex_tab[o >> 23] : status;
+    int *hnp = npts_tab && o >= 0 ? npts_tab[o >> 23] : npts;
+    const bool cnt = on_grid && in_block && o >= 0 && (npts_tab || o < ncells) &&
+                     hst[hc] == EX_DEPOSIT;
     unsigned long long pend = __ballot(cnt);
     while (pend) {
         const int leader = __ffsll((unsigned long long)pend) - 1;
         const int val = __builtin_amdgcn_readlane(o, leader);
         const unsigned long long same = __ballot(cnt && o == val);
-        if (lane == leader) atomicAdd(&npts[val], __popcll(same));
+        if (lane == leader) atomicAdd(&hnp[hc], __popcll(same));
         pend &= ~same;
     }
 }
 
 void launch_owner(const GridP &gp, const HashP &h, const unsigned *start, const float4 *hp,
                   int *owner, int prec, unsigned *ctrs, const int *status, int *npts, int ncells,
-                  hipStream_t s)
+                  int *const *npts_tab, const int *const *stex_tab, hipStream_t s)
 {
-    if (ncells > 0) launch_fill_u32((unsigned *)npts, (size_t)ncells, 0u, s);
+    // one block: its counts start here; several: the caller clears all
+    // blocks' counts before the first owner pass
+    if (!npts_tab && ncells > 0) launch_fill_u32((unsigned *)npts, (size_t)ncells, 0u, s);
     const int tx = (gp.bnum_idx[0] + 3) / 4, ty = (gp.bnum_idx[1] + 3) / 4,
               tz = (gp.bnum_idx[2] + 3) / 4;
     const int ntiles = tx * ty * tz;
     if (ntiles <= 0) return;
     hipLaunchKernelGGL(k_owner, dim3((ntiles + 3) / 4), dim3(256), 0, s, gp, h, start, hp, owner,
-                       prec, ctrs, tx, ty, ntiles, status, npts, ncells);
+                       prec, ctrs, tx, ty, ntiles, status, npts, ncells, npts_tab, stex_tab);
 }
 
 // ---------------------------------------------------------------------------
@@ -457,26 +471,43 @@ void launch_exact_status(const BlockIn &in, const GridP &gp, int *status, int pr
 // k_owner)
 // ---------------------------------------------------------------------------
 
+// With several blocks a point owned by another block's cell is that
+// block's contribution, keyed like a received point (1 << 31 | rank << 24 |
+// cell << 1), so that the fold orders it after this block's own cells.
 __global__ void k_exact_density(GridP gp, const int *__restrict__ owner, const int *__restrict__ status,
-                                const int *__restrict__ npts, int ncells, CellOut o, int prec)
+                                const int *__restrict__ npts, int ncells, CellOut o, int prec,
+                                int *const *__restrict__ npts_tab,
+                                const int *const *__restrict__ stex_tab, int self)
 {
     const size_t n = (size_t)gp.bnum_idx[0] * gp.bnum_idx[1] * gp.bnum_idx[2];
     for (size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x; g < n;
          g += (size_t)gridDim.x * blockDim.x) {
         const int ow = owner[g];
-        if (ow < 0 || ow >= ncells || status[ow] != EX_DEPOSIT) continue;
+        if (ow < 0) continue;
+        int N;
+        unsigned key;
+        if (npts_tab) {
+            const int r = ow >> 23, cell = ow & 0x7fffff;
+            if (stex_tab[r][cell] != EX_DEPOSIT) continue;
+            N = npts_tab[r][cell];
+            key = r == self ? cell_key(cell, 0)
+                            : (0x80000000u | ((unsigned)r << 24) | cell_key(cell, 0));
+        } else {
+            if (ow >= ncells || status[ow] != EX_DEPOSIT) continue;
+            N = npts[ow];
+            key = cell_key(ow, 0);
+        }
         const int lx = (int)(g % gp.bnum_idx[0]);
         const int ly = (int)((g / gp.bnum_idx[0]) % gp.bnum_idx[1]);
         const int lz = (int)(g / ((size_t)gp.bnum_idx[0] * gp.bnum_idx[1]));
-        const int N = npts[ow];
         const float m = prec == 0 ? gp.mass / (float)N : (float)((double)gp.mass / (double)N);
-        deposit_point(gp.bmin_idx[0] + lx, gp.bmin_idx[1] + ly, gp.bmin_idx[2] + lz, m,
-                      cell_key(ow, 0), gp, o);
+        deposit_point(gp.bmin_idx[0] + lx, gp.bmin_idx[1] + ly, gp.bmin_idx[2] + lz, m, key, gp,
+                      o);
     }
 }
 
 __global__ void k_exact_cic(BlockIn in, GridP gp, const int *__restrict__ status_ex,
-                            CellOut o)
+                            CellOut o, const Router *rt, int self)
 {
     const int v = blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= in.ncells) return;
@@ -494,7 +525,7 @@ __global__ void k_exact_cic(BlockIn in, GridP gp, const int *__restrict__ status
             cic_weights(p.x, p.y, p.z, gp.mass, gp, mn, m8);
             for (int k = 0; k < 8; ++k)
                 deposit_point(mn[0] + (k & 1), mn[1] + ((k >> 1) & 1), mn[2] + (k >> 2), m8[k],
-                              cell_key(v, 0), gp, o);
+                              cell_key(v, 0), gp, o, rt, self);
             out_status = 1;
             out_n = 8;
         }
@@ -505,17 +536,18 @@ __global__ void k_exact_cic(BlockIn in, GridP gp, const int *__restrict__ status
 }
 
 void launch_exact_tail(const BlockIn &in, const GridP &gp, const int *owner, const int *status_ex,
-                       const CellOut &o, int prec, hipStream_t s)
+                       const CellOut &o, int prec, int *const *npts_tab,
+                       const int *const *stex_tab, const Router *rt, int self, hipStream_t s)
 {
     const size_t n = (size_t)gp.bnum_idx[0] * gp.bnum_idx[1] * gp.bnum_idx[2];
     size_t blocks = (n + 255) / 256;
     if (blocks > 65536) blocks = 65536;
     if (blocks == 0) blocks = 1;
     hipLaunchKernelGGL(k_exact_density, dim3((unsigned)blocks), dim3(256), 0, s, gp, owner, status_ex,
-                       o.npts, in.ncells, o, prec);
+                       o.npts, in.ncells, o, prec, npts_tab, stex_tab, self);
     if (in.ncells > 0)
         hipLaunchKernelGGL(k_exact_cic, dim3((in.ncells + 255) / 256), dim3(256), 0, s, in, gp,
-                           status_ex, o);
+                           status_ex, o, rt, self);
 }
 
 }  // namespace t2

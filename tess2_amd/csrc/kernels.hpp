@@ -150,15 +150,19 @@ __host__ __forceinline__ int xcd_grid(int n) { return ((n + 7) >> 3) << 3; }
 void launch_circumcenters(const float4 *pts, const int4 *tets, int ntets, float4 *cc,
                           hipStream_t s);
 void launch_hash_build(const float4 *pts, int np, const HashP &h, unsigned *start,
-                       unsigned *fill, unsigned *sums, int *bk, float4 *sorted, hipStream_t s);
+                       unsigned *fill, unsigned *sums, int *bk, float4 *sorted, const int *key,
+                       hipStream_t s);
+// npts_tab / stex_tab (by rank) when several blocks share the assignment,
+// else nullptr
 void launch_owner(const GridP &gp, const HashP &h, const unsigned *start, const float4 *hp,
                   int *owner, int prec, unsigned *ctrs, const int *status, int *npts,
-                  int ncells, hipStream_t s);
+                  int ncells, int *const *npts_tab, const int *const *stex_tab, hipStream_t s);
 // flags: 3 * ncells bytes of scratch (incomplete, outside, defined)
 void launch_exact_status(const BlockIn &in, const GridP &gp, int *status, int prec,
                          unsigned *ctrs, unsigned char *flags, hipStream_t s);
 void launch_exact_tail(const BlockIn &in, const GridP &gp, const int *owner,
-                       const int *status_ex, const CellOut &o, int prec, hipStream_t s);
+                       const int *status_ex, const CellOut &o, int prec, int *const *npts_tab,
+                       const int *const *stex_tab, const Router *rt, int self, hipStream_t s);
 void launch_fill_u32(unsigned *p, size_t n, unsigned v, hipStream_t s);
 // zero every block's density and count grids; circumcenters of every block
 void launch_block_prep(const BlockIn *ins, const CellOut *outs, int nb, int max_tets,

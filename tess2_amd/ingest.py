@@ -41,6 +41,8 @@ class HostBlock:
     bounds_max: np.ndarray         # (3,) float32
     gid: int = 0
     ghost_src: np.ndarray | None = None   # original id of each ghost image
+    ghost_gid: np.ndarray | None = None   # gid of the block holding each ghost as an original
+    ghost_cell: np.ndarray | None = None  # and its index among that block's originals
     info: dict = field(default_factory=dict)
 
     @property
@@ -310,8 +312,14 @@ def decomposed_blocks(points: np.ndarray, block_id: np.ndarray, bounds, ghost_wi
     ghost exchange).  Blocks are tessellated in parallel processes.
     bounds: list of (lo, hi) per block; gid = block index."""
     points = np.ascontiguousarray(points, dtype=np.float32)
+    block_id = np.asarray(block_id)
     w = np.float32(ghost_width)
-    jobs = []
+    # index of each particle among its own block's originals
+    home_idx = np.zeros(points.shape[0], np.int32)
+    for b in range(len(bounds)):
+        sel = block_id == b
+        home_idx[sel] = np.arange(int(sel.sum()), dtype=np.int32)
+    jobs, homes = [], []
     for b, (lo, hi) in enumerate(bounds):
         lo = np.asarray(lo, np.float32)
         hi = np.asarray(hi, np.float32)
@@ -319,6 +327,7 @@ def decomposed_blocks(points: np.ndarray, block_id: np.ndarray, bounds, ghost_wi
         near = (block_id != b) & np.all(points >= lo - w, axis=1) & np.all(points <= hi + w, axis=1)
         allp = np.concatenate([own, points[near]]).astype(np.float32)
         jobs.append((allp, own.shape[0], lo, hi, b))
+        homes.append((block_id[near].astype(np.int32), home_idx[near]))
     if workers > 1 and len(jobs) > 1:
         from concurrent.futures import ProcessPoolExecutor
         with ProcessPoolExecutor(max_workers=workers) as ex:
@@ -326,10 +335,11 @@ def decomposed_blocks(points: np.ndarray, block_id: np.ndarray, bounds, ghost_wi
     else:
         results = [_block_job(j) for j in jobs]
     blocks = []
-    for (allp, norig, lo, hi, b), (tets, dt) in zip(jobs, results):
+    for (allp, norig, lo, hi, b), (tets, dt), (hg, hc) in zip(jobs, results, homes):
         v2t = vert_to_tet(tets, allp.shape[0])
         blocks.append(HostBlock(particles=allp, tets=tets, vert_to_tet=v2t, num_orig=norig,
-                                bounds_min=lo, bounds_max=hi, gid=b, info=dict(qhull_s=dt)))
+                                bounds_min=lo, bounds_max=hi, gid=b, ghost_gid=hg,
+                                ghost_cell=hc, info=dict(qhull_s=dt)))
         if log:
             log(f"  block {b}: {norig} + {allp.shape[0] - norig} ghosts, {tets.shape[0]} tets, "
                 f"{dt:.1f}s")

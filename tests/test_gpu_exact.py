@@ -78,3 +78,50 @@ def test_exact_then_reference_on_one_context(oracle, blocks):
     den = run.fetch(0, ngrid=28 ** 3)["density"]
     want = oracle.dense_single(oracle.Block.from_host(hb), g)["density"]
     assert np.array_equal(den.view(np.uint32), want.view(np.uint32))
+
+
+@pytest.mark.parametrize("prec", [D.MIXED, D.FP64])
+@pytest.mark.parametrize("per_block,ghost,g", [(150, 0.2, 48), (1200, 0.12, 64)])
+def test_exact_assignment_over_blocks(oracle, per_block, ghost, g, prec):
+    """Eight blocks with ghost homes: owners are home keys (rank << 23 |
+    cell), counts are summed over every block that holds a cell's points."""
+    from .test_oracle_exact_multi import home_keys, uniform_blocks
+    say("exact 8 blocks", per_block, g, "mixed" if prec == D.MIXED else "fp64")
+    blocks = uniform_blocks(per_block, 53 + per_block, ghost)
+    run = D.StagedRun(blocks)
+    run.run((g, g, g), assignment=D.EXACT_VORONOI, precision=prec)
+    st = run.sync()
+    want = oracle.exact_dense_multi([oracle.Block.from_host(b) for b in blocks],
+                                    home_keys(blocks), (g, g, g))
+    assert want["cic"] == 0 and st["cells_cic"] == 0
+    worst = 0.0
+    for k in range(len(blocks)):
+        own = run.owner(k)
+        bad = np.nonzero(own != want["owner"][k])[0]
+        assert bad.size == 0, (k, bad.size, bad[:5], own[bad[:5]], want["owner"][k][bad[:5]])
+        s, npts = run.cells(k)
+        assert np.array_equal(s, want["status"][k])
+        assert np.array_equal(npts, want["npts"][k])
+        den = run.fetch(k)["density"]
+        w = want["density"][k]
+        assert np.array_equal(den == 0, w == 0)
+        nz = w != 0
+        if nz.any():
+            worst = max(worst, float((np.abs(den[nz] - w[nz]) / np.abs(w[nz])).max()))
+    assert worst <= REL_TOL, worst
+    ncells = sum(int((s == 0).sum()) for s in want["status"])
+    assert st["tot_mass"] == pytest.approx(ncells, rel=1e-5)
+    say("  fallbacks", st["fp64_fallbacks"], "max rel", worst)
+
+
+def test_exact_over_blocks_needs_ghost_homes():
+    from .test_oracle_exact_multi import uniform_blocks
+    blocks = uniform_blocks(150, 60, 0.2)
+    for b in blocks:
+        b.ghost_gid = b.ghost_cell = None
+    run = D.StagedRun(blocks)
+    with pytest.raises(D.Tess2Error) as e:
+        run.run((24, 24, 24), assignment=D.EXACT_VORONOI)
+    assert "ghost homes" in str(e.value)
+    run.run((24, 24, 24))                  # the reference scan needs none
+    run.sync()

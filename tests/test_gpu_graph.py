@@ -141,3 +141,9 @@ def test_options_and_autotune_keep_results(oracle, multi):
     assert ms > 0
     for i in range(len(multi)):
         assert np.array_equal(_bits(run.fetch(i)["density"]), got[i]), i
+
+
+def test_graph_replay_exact_multiblock(multi):
+    say("graph exact multiblock")
+    kw = dict(glo_num_idx=(33, 33, 33), assignment=D.EXACT_VORONOI)
+    _replays(multi, kw, exact=True)

@@ -78,11 +78,18 @@ def setup(cfg, load=None):
     p = synthetic.config_particles(cfg)
     g, src = ingest.periodic_ghosts(p, c["ghost"] / c["n_side"])
     allp = np.concatenate([p, g]).astype(np.float32)
+    q64 = None
+    if c["kind"] == "lattice":
+        # the triangulation is decided on the jitter scaled 10^4x (near-cospherical
+        # float32 cubes are beyond Qhull's own precision handling)
+        t64 = synthetic.lattice_tiebreak(
+            synthetic.jittered_lattice64(c["n_side"], c["jitter"], seed=cfg), c["n_side"])
+        q64 = ingest.ghost_positions64(p, t64, g, src)
     log(f"[setup] config {cfg} ({c['name']}): {p.shape[0]} particles + {g.shape[0]} ghosts "
         f"({time.time() - t0:.1f}s)")
     t1 = time.time()
     hb = ingest.make_block(allp, p.shape[0], (0, 0, 0), (1, 1, 1), workers=workers,
-                           ghost_src=src, log=log, closed_first=p.shape[0])
+                           ghost_src=src, log=log, closed_first=p.shape[0], qhull_points=q64)
     log(f"[setup] Delaunay: {hb.num_tets} tets with {workers} workers "
         f"({time.time() - t1:.1f}s)")
     return c, [hb]

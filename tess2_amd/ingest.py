@@ -68,11 +68,16 @@ def vert_to_tet(tets: np.ndarray, num_particles: int) -> np.ndarray:
     return v2t
 
 
-def qhull_tets(points: np.ndarray) -> np.ndarray:
+def qhull_tets(points: np.ndarray, qhull_points: np.ndarray | None = None) -> np.ndarray:
     """Delaunay tets of float32 points as an (T, 8) int32 tet_t array.
-    Points are promoted to double as the reference does (src/tess-qhull.c:39-42)."""
+    Points are promoted to double as the reference does (src/tess-qhull.c:39-42).
+    qhull_points: float64 positions of the same particles that decide the
+    triangulation instead (for float32 sets too degenerate for Qhull)."""
     from scipy.spatial import Delaunay
-    p64 = np.ascontiguousarray(points, dtype=np.float32).astype(np.float64)
+    if qhull_points is not None:
+        p64 = np.ascontiguousarray(qhull_points, dtype=np.float64)
+    else:
+        p64 = np.ascontiguousarray(points, dtype=np.float32).astype(np.float64)
     d = Delaunay(p64, qhull_options="Qt")
     T = d.simplices.shape[0]
     tets = np.empty((T, 8), np.int32)
@@ -133,7 +138,9 @@ def _column_job(args):
     point cloud's bounding box, lies inside the padded column: no point
     outside the padded column can then lie in the sphere, so the tet is
     Delaunay for the whole set."""
-    (pts, lo, hi, margin, cloud_lo, cloud_hi) = args
+    (pts, lo, hi, margin, cloud_lo, cloud_hi, q64) = args
+    if q64 is not None:
+        pts = q64          # the float64 positions decide membership too
     pad_lo = lo - margin
     pad_hi = hi + margin
     inside = np.ones(pts.shape[0], bool)
@@ -142,7 +149,7 @@ def _column_job(args):
     sel = np.nonzero(inside)[0]
     sub = pts[sel]
     t0 = time.time()
-    tets = qhull_tets(sub)
+    tets = qhull_tets(sub, None if q64 is None else sub)
     tv = sel[tets[:, :4]]                        # global vertex ids
     ov = pts[tv.min(1)]
     own = np.ones(tv.shape[0], bool)
@@ -202,35 +209,40 @@ def open_star_vertices(tets: np.ndarray, num_particles: int) -> np.ndarray:
 
 
 def tessellate(points: np.ndarray, workers: int = 1, columns: int | None = None,
-               log=None, min_points: int = 200_000, closed_first: int = 0) -> np.ndarray:
+               log=None, min_points: int = 200_000, closed_first: int = 0,
+               qhull_points: np.ndarray | None = None) -> np.ndarray:
     """Delaunay tets (T, 8) of all points.  With workers > 1 the points are
     split into x-y columns tessellated in parallel processes and merged into
     one tessellation (tet numbering then follows the columns).  Only tets
     certified Delaunay for the whole set are kept; the first `closed_first`
     points (the original particles of a ghosted block) must end up with
     closed stars, otherwise the margin doubles and the columns are redone.
-    Degenerate input whose columns cannot be merged falls back to one call."""
+    Degenerate input whose columns cannot be merged falls back to one call.
+    qhull_points: float64 positions that decide the triangulation (see
+    qhull_tets); the columns are then cut and certified on them as well."""
     points = np.ascontiguousarray(points, dtype=np.float32)
+    q64 = None if qhull_points is None else np.ascontiguousarray(qhull_points, np.float64)
     n = points.shape[0]
     if workers <= 1 or n < min_points:
-        return qhull_tets(points)
+        return qhull_tets(points, q64)
     from concurrent.futures import ProcessPoolExecutor
     cols = columns or workers
     nx = int(np.floor(np.sqrt(cols)))
     while cols % nx:
         nx -= 1
     ny = cols // nx
-    qx = np.quantile(points[:, 0], np.linspace(0, 1, nx + 1))
-    qy = np.quantile(points[:, 1], np.linspace(0, 1, ny + 1))
+    geo = points if q64 is None else q64
+    qx = np.quantile(geo[:, 0], np.linspace(0, 1, nx + 1))
+    qy = np.quantile(geo[:, 1], np.linspace(0, 1, ny + 1))
     qx[0], qx[-1], qy[0], qy[-1] = -np.inf, np.inf, -np.inf, np.inf
-    cloud_lo = points.min(0).astype(np.float64)
-    cloud_hi = points.max(0).astype(np.float64)
+    cloud_lo = geo.min(0).astype(np.float64)
+    cloud_hi = geo.max(0).astype(np.float64)
     extent = float((cloud_hi - cloud_lo).max())
     spacing = extent / max(n, 1) ** (1.0 / 3.0)
     margin = 3.0 * spacing
     for attempt in range(5):
         jobs = [(points, np.array([qx[i], qy[j]]), np.array([qx[i + 1], qy[j + 1]]),
-                 margin, cloud_lo, cloud_hi) for i in range(nx) for j in range(ny)]
+                 margin, cloud_lo, cloud_hi, q64) for i in range(nx) for j in range(ny)]
         parts, dropped = [], 0
         with ProcessPoolExecutor(max_workers=workers) as ex:
             for k, (tv, ok, dt, m) in enumerate(ex.map(_column_job, jobs)):
@@ -248,7 +260,7 @@ def tessellate(points: np.ndarray, workers: int = 1, columns: int | None = None,
             # pieces do not fit; one Qhull call decides consistently
             if log:
                 log("  columns disagree on degenerate (cospherical) cells; one Qhull call")
-            return qhull_tets(points)
+            return qhull_tets(points, q64)
         tets = np.empty((tv.shape[0], 8), np.int32)
         tets[:, :4] = tv
         tets[:, 4:] = nb
@@ -269,10 +281,11 @@ def tessellate(points: np.ndarray, workers: int = 1, columns: int | None = None,
 
 def make_block(particles: np.ndarray, num_orig: int, bounds_min, bounds_max,
                workers: int = 1, gid: int = 0, ghost_src=None, log=None,
-               closed_first: int = 0) -> HostBlock:
+               closed_first: int = 0, qhull_points=None) -> HostBlock:
     particles = np.ascontiguousarray(particles, dtype=np.float32)
     t0 = time.time()
-    tets = tessellate(particles, workers=workers, log=log, closed_first=closed_first)
+    tets = tessellate(particles, workers=workers, log=log, closed_first=closed_first,
+                      qhull_points=qhull_points)
     t1 = time.time()
     v2t = vert_to_tet(tets, particles.shape[0])
     return HostBlock(particles=particles, tets=tets, vert_to_tet=v2t, num_orig=num_orig,
@@ -281,14 +294,23 @@ def make_block(particles: np.ndarray, num_orig: int, bounds_min, bounds_max,
                      ghost_src=ghost_src, info=dict(qhull_s=t1 - t0))
 
 
+def ghost_positions64(p: np.ndarray, p64: np.ndarray, g: np.ndarray, src: np.ndarray):
+    """float64 positions of the periodic images g of p (sources src), from
+    the float64 positions p64 of the originals: the same whole-box shift."""
+    shift = np.rint(g.astype(np.float64) - p[src].astype(np.float64))
+    return np.concatenate([p64, p64[src] + shift])
+
+
 def periodic_block(p: np.ndarray, ghost_width: float, workers: int = 1,
-                   log=None) -> HostBlock:
-    """One periodic unit-cube block: originals + periodic ghost images."""
+                   log=None, p64: np.ndarray | None = None) -> HostBlock:
+    """One periodic unit-cube block: originals + periodic ghost images.
+    p64: float64 positions of p that decide the triangulation (qhull_tets)."""
     p = np.ascontiguousarray(p, dtype=np.float32)
     g, src = periodic_ghosts(p, ghost_width)
     allp = np.concatenate([p, g]).astype(np.float32)
+    q64 = None if p64 is None else ghost_positions64(p, p64, g, src)
     return make_block(allp, p.shape[0], (0, 0, 0), (1, 1, 1), workers=workers,
-                      ghost_src=src, log=log, closed_first=p.shape[0])
+                      ghost_src=src, log=log, closed_first=p.shape[0], qhull_points=q64)
 
 
 def default_workers() -> int:

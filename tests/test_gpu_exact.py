@@ -32,6 +32,11 @@ def blocks():
     # (points at multiples of 1/16): many grid points exactly equidistant
     # from 2, 4 or 8 sites
     out["lattice"] = ingest.periodic_block(synthetic.jittered_lattice(8, 0.0, seed=4), 3.0 / 8)
+    # lattice + 1e-6-spacing jitter (float32 mostly on the lattice) on a grid
+    # of 4 points per spacing: grid points equidistant from lattice sites
+    p64 = synthetic.jittered_lattice64(20, 1e-6, seed=4)
+    out["jitter"] = ingest.periodic_block(synthetic.jittered_lattice(20, 1e-6, seed=4), 3.0 / 20,
+                                          p64=synthetic.lattice_tiebreak(p64, 20))
     return out
 
 
@@ -57,7 +62,8 @@ def _check(oracle, hb, g, prec):
 
 
 @pytest.mark.parametrize("prec", [D.MIXED, D.FP64])
-@pytest.mark.parametrize("name,g", [("grf", 40), ("uniform", 32), ("lattice", 17)])
+@pytest.mark.parametrize("name,g", [("grf", 40), ("uniform", 32), ("lattice", 17),
+                                    ("jitter", 81)])
 def test_exact_assignment(oracle, blocks, name, g, prec):
     say("exact", name, g, "mixed" if prec == D.MIXED else "fp64")
     st, rel = _check(oracle, blocks[name], g, prec)

@@ -50,10 +50,16 @@ def blocks():
     out["grf"] = ingest.periodic_block(synthetic.grf_lattice(24, 0.3, seed=5), 4.0 / 24)
     # exact lattice: Qhull's Qt output has flat tets with NaN circumcenters
     out["lattice"] = ingest.periodic_block(synthetic.jittered_lattice(8, 0.0, seed=4), 3.0 / 8)
+    # lattice + 1e-6-spacing jitter, mostly rounded away in float32; the
+    # triangulation is decided on the jitter scaled 10^4x (near-cospherical
+    # cubes: slivers with huge or undefined circumcenters)
+    p64 = synthetic.jittered_lattice64(20, 1e-6, seed=4)
+    out["jitter"] = ingest.periodic_block(synthetic.jittered_lattice(20, 1e-6, seed=4), 3.0 / 20,
+                                          p64=synthetic.lattice_tiebreak(p64, 20))
     return out
 
 
-@pytest.mark.parametrize("name", ["periodic4k", "p2", "grf", "lattice"])
+@pytest.mark.parametrize("name", ["periodic4k", "p2", "grf", "lattice", "jitter"])
 def test_circumcenters_bitexact(oracle, blocks, name):
     hb = blocks[name]
     say("circumcenters", name, hb.num_tets)
@@ -62,7 +68,7 @@ def test_circumcenters_bitexact(oracle, blocks, name):
     assert bits_equal(got, want), first_diff(got, want)
 
 
-@pytest.mark.parametrize("name", ["periodic4k", "p2", "grf", "lattice"])
+@pytest.mark.parametrize("name", ["periodic4k", "p2", "grf", "lattice", "jitter"])
 def test_volumes_bitexact(oracle, blocks, name):
     hb = blocks[name]
     say("volumes", name, hb.num_orig)
@@ -78,7 +84,7 @@ def _oracle_dense(oracle, hb, g, cic=False, mass=1.0):
 
 @pytest.mark.parametrize("name,g", [("periodic4k", 32), ("periodic4k", 64), ("p2", 64),
                                     ("periodic4k", 8), ("grf", 40), ("lattice", 17),
-                                    ("lattice", 33)])
+                                    ("lattice", 33), ("jitter", 80)])
 def test_dense_reference_scan_bitexact(oracle, blocks, name, g):
     hb = blocks[name]
     say("dense", name, g)

@@ -56,6 +56,8 @@ struct DevBlock {
     int *hkey = nullptr;                     // [np] home key of each particle (several blocks)
     std::vector<int> ghost_gid, ghost_cell;  // home of each ghost, when given
     size_t owner_cap = 0;
+    unsigned *defq = nullptr;                // FP32 filter failures awaiting the FP64 search
+    unsigned defq_cap = 0;
     BlockStats *st = nullptr;                // this block's record in the context's array
     unsigned *ctrs = nullptr;                // st->ctrs
     BlockStats *hm = nullptr;                // its pinned host copy, refreshed by every run
@@ -195,7 +197,8 @@ static void free_block(DevBlock &b)
     void *ps[] = {b.order,  b.pts,    b.tets,   b.v2t,   b.cc,   b.vol,  b.status,    b.npts,
                   b.density, b.count, b.first, b.head, b.ovf,  b.slow,
                   b.owner, b.status_ex, b.exflags, b.hstart, b.hfill,
-                  b.hsums,  b.hbk,    b.hsorted, b.hkey, b.seg, b.fold[0], b.fold[1], b.fold[2]};
+                  b.hsums,  b.hbk,    b.hsorted, b.hkey, b.defq, b.seg, b.fold[0], b.fold[1],
+                  b.fold[2]};
     for (void *p : ps)
         if (p) (void)hipFree(p);
     b = DevBlock();
@@ -761,6 +764,17 @@ static int prepare_run(tess2_ctx *c, const RunArgs &a)
             r = dalloc(c, &d.owner, n);
             if (r) return r;
             d.owner_cap = n;
+            // a quarter of the grid (beyond it the search runs inline); the
+            // queue holds 32-bit grid indices
+            if (d.defq) (void)hipFree(d.defq);
+            d.defq = nullptr;
+            d.defq_cap = 0;
+            if (n < (1ull << 32)) {
+                const size_t cap = n / 4 + 1024;
+                r = dalloc(c, &d.defq, cap);
+                if (r) return r;
+                d.defq_cap = (unsigned)cap;
+            }
         }
         c->h_ins[k] = make_in(d);
         c->h_gps[k] = gp;
@@ -833,7 +847,7 @@ static int enqueue_kernels(tess2_ctx *c, const RunArgs &a)
         for (int k = 0; k < nb; ++k) {
             DevBlock &d = c->blocks[k];
             launch_owner(c->h_gps[k], d.hash, d.hstart, d.hsorted, d.owner, a.precision, d.ctrs,
-                         d.status_ex, d.npts, d.norig, ntab, stab, s);
+                         d.status_ex, d.npts, d.norig, ntab, stab, d.defq, d.defq_cap, s);
         }
         if (ev) HIPCHK(c, hipEventRecord(c->ev_k1, s));
         // a cell's count changes in its CIC pass only when it owns no point,
@@ -1020,7 +1034,7 @@ extern "C" int tess2_sync(tess2_ctx *c, tess2_stats_t *st)
             st->points_deposited += (long long)d.hm->cstat[5] - d.hm->ctrs[C_OFFBLOCK] -
                                     d.hm->ctrs[C_ROUTED];
             st->points_shared += d.hm->ctrs[C_OVF];
-            st->fp64_fallbacks += d.hm->ctrs[C_FALLBACK];
+            st->fp64_fallbacks += d.hm->ctrs[C_FALLBACK] + d.hm->ctrs[C_DEFER];
             st->cells_deferred += d.hm->ctrs[C_SLOW];
             st->tot_mass += d.hm->gsum * (double)d.gp.div;
             float mx;

@@ -289,7 +289,8 @@ __global__ __launch_bounds__(256) void k_owner(GridP gp, HashP h,
                                                int ntiles, const int *__restrict__ status,
                                                int *__restrict__ npts, int ncells,
                                                int *const *__restrict__ npts_tab,
-                                               const int *const *__restrict__ stex_tab)
+                                               const int *const *__restrict__ stex_tab,
+                                               unsigned *__restrict__ defq, unsigned defq_cap)
 {
     const int wave = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (wave >= ntiles) return;
@@ -309,13 +310,31 @@ __global__ __launch_bounds__(256) void k_owner(GridP gp, HashP h,
                           py <= gp.bmax[1] && pz >= gp.bmin[2] && pz <= gp.bmax[2];
     if (!in_block && on_grid) owner[lidx] = -1;
     int o = -1;
+    bool failed = false;
     if (in_block && on_grid) {
         if (prec == 0) o = nearest_f32(px, py, pz, h, start, hp);
-        if (o < 0) {
-            if (prec == 0) atomicAdd(&ctrs[C_FALLBACK], 1u);
-            o = nearest_f64(px, py, pz, h, start, hp);
+        failed = o < 0 && prec == 0;
+    }
+    // FP32 filter failures go to a queue that k_owner_deferred searches in
+    // FP64 with full waves: searched here, one failing lane would hold its
+    // 63 neighbours for a whole FP64 search.  C_DEFER counts every failure
+    // (one atomic per wave); beyond the queue's end the search runs inline.
+    bool defer = false;
+    const unsigned long long dm = __ballot(failed);
+    if (dm) {
+        const int leader = __ffsll(dm) - 1;
+        unsigned base = 0;
+        if (lane == leader) base = atomicAdd(&ctrs[C_DEFER], (unsigned)__popcll(dm));
+        base = __shfl(base, leader);
+        const unsigned slot = base + (unsigned)__popcll(dm & ((1ull << lane) - 1ull));
+        if (failed && slot < defq_cap) {
+            defq[slot] = (unsigned)lidx;
+            defer = true;
         }
-        owner[lidx] = o;
+    }
+    if (in_block && on_grid) {
+        if (o < 0 && !defer) o = nearest_f64(px, py, pz, h, start, hp);
+        owner[lidx] = o;           // -1 for a deferred point until its FP64 search
     }
     // points per deposited cell; the lanes of a tile mostly share owners, so
     // one atomic per distinct owner in the wave
@@ -334,9 +353,43 @@ __global__ __launch_bounds__(256) void k_owner(GridP gp, HashP h,
     }
 }
 
+// the FP64 search of the points k_owner deferred, one lane per point (the
+// queue holds the points of each wave together, so lanes stay close)
+__global__ __launch_bounds__(256) void k_owner_deferred(GridP gp, HashP h,
+                                                        const unsigned *__restrict__ start,
+                                                        const float4 *__restrict__ hp,
+                                                        int *__restrict__ owner,
+                                                        const unsigned *ctrs,
+                                                        const unsigned *__restrict__ defq,
+                                                        unsigned defq_cap,
+                                                        const int *__restrict__ status,
+                                                        int *__restrict__ npts, int ncells,
+                                                        int *const *__restrict__ npts_tab,
+                                                        const int *const *__restrict__ stex_tab)
+{
+    unsigned n = ctrs[C_DEFER];
+    if (n > defq_cap) n = defq_cap;
+    const unsigned nx = (unsigned)gp.bnum_idx[0], ny = (unsigned)gp.bnum_idx[1];
+    for (unsigned i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const unsigned lidx = defq[i];
+        const int lx = (int)(lidx % nx), ly = (int)((lidx / nx) % ny), lz = (int)(lidx / (nx * ny));
+        const float px = idx2phys1(gp.bmin_idx[0] + lx, gp.step[0], gp.gmin[0]);
+        const float py = idx2phys1(gp.bmin_idx[1] + ly, gp.step[1], gp.gmin[1]);
+        const float pz = idx2phys1(gp.bmin_idx[2] + lz, gp.step[2], gp.gmin[2]);
+        const int o = nearest_f64(px, py, pz, h, start, hp);
+        owner[lidx] = o;
+        if (o < 0) continue;
+        const int hc = npts_tab ? (o & 0x7fffff) : o;
+        const int *hst = npts_tab ? stex_tab[o >> 23] : status;
+        int *hnp = npts_tab ? npts_tab[o >> 23] : npts;
+        if ((npts_tab || o < ncells) && hst[hc] == EX_DEPOSIT) atomicAdd(&hnp[hc], 1);
+    }
+}
+
 void launch_owner(const GridP &gp, const HashP &h, const unsigned *start, const float4 *hp,
                   int *owner, int prec, unsigned *ctrs, const int *status, int *npts, int ncells,
-                  int *const *npts_tab, const int *const *stex_tab, hipStream_t s)
+                  int *const *npts_tab, const int *const *stex_tab, unsigned *defq,
+                  unsigned defq_cap, hipStream_t s)
 {
     // one block: its counts start here; several: the caller clears all
     // blocks' counts before the first owner pass
@@ -346,7 +399,14 @@ void launch_owner(const GridP &gp, const HashP &h, const unsigned *start, const 
     const int ntiles = tx * ty * tz;
     if (ntiles <= 0) return;
     hipLaunchKernelGGL(k_owner, dim3((ntiles + 3) / 4), dim3(256), 0, s, gp, h, start, hp, owner,
-                       prec, ctrs, tx, ty, ntiles, status, npts, ncells, npts_tab, stex_tab);
+                       prec, ctrs, tx, ty, ntiles, status, npts, ncells, npts_tab, stex_tab,
+                       defq, prec == 0 ? defq_cap : 0u);
+    if (prec == 0 && defq_cap > 0) {
+        const unsigned nblk = defq_cap / 256 + 1;
+        hipLaunchKernelGGL(k_owner_deferred, dim3(nblk < 2048 ? nblk : 2048), dim3(256), 0, s,
+                           gp, h, start, hp, owner, ctrs, defq, defq_cap, status, npts, ncells,
+                           npts_tab, stex_tab);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -381,6 +441,10 @@ __global__ void k_exact_flags(BlockIn in, GridP gp, int prec, unsigned char *__r
         lo[i] = gp.dmin[i] - e;
         hi[i] = gp.dmax[i] + e;
     }
+    __shared__ unsigned s_fb;
+    if (threadIdx.x == 0) s_fb = 0;
+    __syncthreads();
+    unsigned nfb = 0;   // FP64 re-checks of this lane
     for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < in.ntets;
          t += gridDim.x * blockDim.x) {
         const int4 vv = in.tets[2 * t];
@@ -413,9 +477,7 @@ __global__ void k_exact_flags(BlockIn in, GridP gp, int prec, unsigned char *__r
                     is_out = surely_out;
                 }
             }
-            const unsigned long long fb = __ballot(!decided);
-            if ((threadIdx.x & 63) == (unsigned)(__ffsll((unsigned long long)__ballot(1)) - 1) && fb)
-                atomicAdd(&ctrs[C_FALLBACK], (unsigned)__popcll(fb));
+            nfb += decided ? 0u : 1u;
         }
         if (!decided) {
             const float4 a = in.pts[vv.x], b = in.pts[vv.y], c = in.pts[vv.z], d = in.pts[vv.w];
@@ -438,6 +500,14 @@ __global__ void k_exact_flags(BlockIn in, GridP gp, int prec, unsigned char *__r
             }
         }
     }
+    // one global atomic per workgroup (and few workgroups): same-address
+    // atomics serialize in L2, ~10 ns each, and one per wave outlasted the
+    // kernel's own work
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) nfb += __shfl_xor(nfb, m);
+    if ((threadIdx.x & 63) == 0 && nfb) atomicAdd(&s_fb, nfb);
+    __syncthreads();
+    if (threadIdx.x == 0 && s_fb) atomicAdd(&ctrs[C_FALLBACK], s_fb);
 }
 
 __global__ void k_exact_classify(const int *__restrict__ v2t, int nc,
@@ -459,7 +529,7 @@ void launch_exact_status(const BlockIn &in, const GridP &gp, int *status, int pr
     if (in.ncells <= 0) return;
     launch_fill_u32((unsigned *)flags, ((size_t)in.ncells * 3 + 3) / 4, 0u, s);
     if (in.ntets > 0) {
-        const int blocks = std::min(32768, (in.ntets + 255) / 256);
+        const int blocks = std::min(4096, (in.ntets + 255) / 256);
         hipLaunchKernelGGL(k_exact_flags, dim3(blocks), dim3(256), 0, s, in, gp, prec, flags, ctrs);
     }
     hipLaunchKernelGGL(k_exact_classify, dim3((in.ncells + 255) / 256), dim3(256), 0, s, in.v2t,

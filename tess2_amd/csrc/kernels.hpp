@@ -30,7 +30,7 @@ enum {
     C_SLOW = 2,       // cells queued for the large-capacity pass
     C_ERR = 3,        // bit 0: deferred-cell list full; bit 2: cell beyond all capacities;
                       // bit 3: a shared point beyond the fold's capacities
-    C_FREE4 = 4,      // (free)
+    C_DEFER = 4,      // grid points queued for the FP64 search of the exact assignment
     C_FALLBACK = 5,   // FP64 re-checks of the exact assignment
     C_ROUTED = 6,     // points deposited into another block's grid
     C_SEG = 7,        // contributions gathered into per-point segments for the fold
@@ -153,10 +153,12 @@ void launch_hash_build(const float4 *pts, int np, const HashP &h, unsigned *star
                        unsigned *fill, unsigned *sums, int *bk, float4 *sorted, const int *key,
                        hipStream_t s);
 // npts_tab / stex_tab (by rank) when several blocks share the assignment,
-// else nullptr
+// else nullptr; defq: queue of defq_cap grid points for the FP64 re-search
+// of the mixed precision (0: searched inline)
 void launch_owner(const GridP &gp, const HashP &h, const unsigned *start, const float4 *hp,
                   int *owner, int prec, unsigned *ctrs, const int *status, int *npts,
-                  int ncells, int *const *npts_tab, const int *const *stex_tab, hipStream_t s);
+                  int ncells, int *const *npts_tab, const int *const *stex_tab, unsigned *defq,
+                  unsigned defq_cap, hipStream_t s);
 // flags: 3 * ncells bytes of scratch (incomplete, outside, defined)
 void launch_exact_status(const BlockIn &in, const GridP &gp, int *status, int prec,
                          unsigned *ctrs, unsigned char *flags, hipStream_t s);

@@ -186,7 +186,9 @@ int tess2_fetch(tess2_ctx *ctx, int b, float *density, float *volumes,
                 float *circumcenters);
 
 /* Nearest-site owner of every grid point of block b from the last
- * EXACT_VORONOI run (-1 outside the block bounds); block grid order. */
+ * EXACT_VORONOI run (-1 outside the block bounds); block grid order.  One
+ * staged block: the particle index; several: the home key
+ * (rank << 23 | cell, rank = position in ascending gid order). */
 int tess2_fetch_owner(tess2_ctx *ctx, int b, int *owner);
 
 /* Average device time per launch (ms) of the dominant kernel over the

@@ -850,12 +850,16 @@ static int enqueue_kernels(tess2_ctx *c, const RunArgs &a)
                          d.status_ex, d.npts, d.norig, ntab, stab, d.defq, d.defq_cap, s);
         }
         if (ev) HIPCHK(c, hipEventRecord(c->ev_k1, s));
-        // a cell's count changes in its CIC pass only when it owns no point,
-        // so no density pass reads a count another block's CIC pass rewrote
-        for (int k = 0; k < nb; ++k)
-            launch_exact_tail(c->h_ins[k], c->h_gps[k], c->blocks[k].owner,
-                              c->blocks[k].status_ex, c->h_outs[k], a.precision, ntab, stab,
-                              multi ? c->d_router : nullptr, k, s);
+        // every density pass before any CIC pass: an owned point's density is
+        // its first contribution, stored without atomics, and CIC corners
+        // routed to another block must come after it.  A cell's count changes
+        // in its CIC pass only when it owns no point, so no density pass
+        // would read a rewritten count anyway.
+        for (int part = 1; part <= 2; ++part)
+            for (int k = 0; k < nb; ++k)
+                launch_exact_tail(c->h_ins[k], c->h_gps[k], c->blocks[k].owner,
+                                  c->blocks[k].status_ex, c->h_outs[k], a.precision, ntab, stab,
+                                  multi ? c->d_router : nullptr, k, part, s);
     } else if (a.alg == TESS2_DENSE_CIC) {
         if (ev) HIPCHK(c, hipEventRecord(c->ev_k0, s));
         for (int k = 0; k < nb; ++k)

@@ -567,12 +567,13 @@ __global__ void k_exact_density(GridP gp, const int *__restrict__ owner, const i
             N = npts[ow];
             key = cell_key(ow, 0);
         }
-        const int lx = (int)(g % gp.bnum_idx[0]);
-        const int ly = (int)((g / gp.bnum_idx[0]) % gp.bnum_idx[1]);
-        const int lz = (int)(g / ((size_t)gp.bnum_idx[0] * gp.bnum_idx[1]));
         const float m = prec == 0 ? gp.mass / (float)N : (float)((double)gp.mass / (double)N);
-        deposit_point(gp.bmin_idx[0] + lx, gp.bmin_idx[1] + ly, gp.bmin_idx[2] + lz, m, key, gp,
-                      o);
+        // an owned point is in the block and its grid, and this is its first
+        // contribution (every density pass runs before any CIC pass): the
+        // first-contribution stores of push_point without its atomic
+        o.count[g] = 1u;
+        o.first[g] = make_uint2(key, __float_as_uint(m));
+        o.density[g] = (float)((double)m / (double)gp.div);
     }
 }
 
@@ -607,15 +608,17 @@ __global__ void k_exact_cic(BlockIn in, GridP gp, const int *__restrict__ status
 
 void launch_exact_tail(const BlockIn &in, const GridP &gp, const int *owner, const int *status_ex,
                        const CellOut &o, int prec, int *const *npts_tab,
-                       const int *const *stex_tab, const Router *rt, int self, hipStream_t s)
+                       const int *const *stex_tab, const Router *rt, int self, int parts,
+                       hipStream_t s)
 {
     const size_t n = (size_t)gp.bnum_idx[0] * gp.bnum_idx[1] * gp.bnum_idx[2];
     size_t blocks = (n + 255) / 256;
     if (blocks > 65536) blocks = 65536;
     if (blocks == 0) blocks = 1;
-    hipLaunchKernelGGL(k_exact_density, dim3((unsigned)blocks), dim3(256), 0, s, gp, owner, status_ex,
-                       o.npts, in.ncells, o, prec, npts_tab, stex_tab, self);
-    if (in.ncells > 0)
+    if (parts & 1)
+        hipLaunchKernelGGL(k_exact_density, dim3((unsigned)blocks), dim3(256), 0, s, gp, owner,
+                           status_ex, o.npts, in.ncells, o, prec, npts_tab, stex_tab, self);
+    if ((parts & 2) && in.ncells > 0)
         hipLaunchKernelGGL(k_exact_cic, dim3((in.ncells + 255) / 256), dim3(256), 0, s, in, gp,
                            status_ex, o, rt, self);
 }

@@ -162,9 +162,12 @@ void launch_owner(const GridP &gp, const HashP &h, const unsigned *start, const 
 // flags: 3 * ncells bytes of scratch (incomplete, outside, defined)
 void launch_exact_status(const BlockIn &in, const GridP &gp, int *status, int prec,
                          unsigned *ctrs, unsigned char *flags, hipStream_t s);
+// parts: 1 = densities of the owned points, 2 = CIC of cells that own none;
+// every block's part 1 must precede any block's part 2
 void launch_exact_tail(const BlockIn &in, const GridP &gp, const int *owner,
                        const int *status_ex, const CellOut &o, int prec, int *const *npts_tab,
-                       const int *const *stex_tab, const Router *rt, int self, hipStream_t s);
+                       const int *const *stex_tab, const Router *rt, int self, int parts,
+                       hipStream_t s);
 void launch_fill_u32(unsigned *p, size_t n, unsigned v, hipStream_t s);
 // zero every block's density and count grids; circumcenters of every block
 void launch_block_prep(const BlockIn *ins, const CellOut *outs, int nb, int max_tets,

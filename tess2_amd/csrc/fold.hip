@@ -40,10 +40,16 @@ __device__ __forceinline__ unsigned fold_capacity(const CellOut &o, int cls)
 
 // 1. one segment per shared point; slot and segment reservations are
 // aggregated per workgroup (one atomic per workgroup and counter per pass)
-__global__ __launch_bounds__(256) void k_fold_gather(const CellOut *__restrict__ outs)
+// A point with exactly two contributions (the common case: a point on the
+// face between two cells) is folded here at once: its representative entry
+// is its only list entry, and the pair is added in the order the small fold
+// would give it (the list entry first unless its key is larger).
+__global__ __launch_bounds__(256) void k_fold_gather(const CellOut *__restrict__ outs,
+                                                     const GridP *__restrict__ gps)
 {
     __shared__ unsigned s_cnt[4][4];   // per wave: points of class 0..2, segment total
     const CellOut &o = outs[blockIdx.y];
+    const double dd = (double)gps[blockIdx.y].div;
     if (fold_overflowed(o)) return;
     const unsigned n = o.ctrs[C_OVF];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -54,11 +60,28 @@ __global__ __launch_bounds__(256) void k_fold_gather(const CellOut *__restrict__
         int lidx = 0, cls = 0;
         unsigned cnt = 0;
         if (e < n) {
-            lidx = o.ovf[e].x;
+            const int4 r = o.ovf[e];
+            lidx = r.x;
             mine = o.head[lidx] == (int)e;   // the entry that represents its point
             if (mine) {
                 cnt = o.count[lidx];
-                cls = cnt <= FOLD_SMALL_MAX ? 0 : (cnt <= FOLD_MID_MAX ? 1 : 2);
+                if (cnt == 2u) {
+                    const uint2 f = o.first[lidx];
+                    unsigned k0 = (unsigned)r.y, m0 = (unsigned)r.z, k1 = f.x, m1 = f.y;
+                    if (k0 > k1) {
+                        k0 = f.x;
+                        m0 = f.y;
+                        m1 = (unsigned)r.z;
+                    }
+                    float rho = (float)(0.0 + (double)__uint_as_float(m0) / dd);
+                    rho = (float)((double)rho + (double)__uint_as_float(m1) / dd);
+                    o.density[lidx] = rho;
+                    o.head[lidx] = -(1 + 3);          // class 3: nothing to scatter
+                    mine = false;
+                    cnt = 0;
+                } else {
+                    cls = cnt <= FOLD_SMALL_MAX ? 0 : (cnt <= FOLD_MID_MAX ? 1 : 2);
+                }
             }
         }
         const unsigned long long m0 = __ballot(mine && cls == 0);
@@ -381,7 +404,7 @@ void launch_fold(const CellOut *outs, const GridP *gps, int nb, unsigned max_ovf
     // block when there are many blocks (the loops stride)
     const unsigned per = nb > 16 ? 256 : 4096;
     const unsigned b = capped_blocks(max_ovf_cap, 256, per);
-    hipLaunchKernelGGL(k_fold_gather, dim3(b, nb), dim3(256), 0, s, outs);
+    hipLaunchKernelGGL(k_fold_gather, dim3(b, nb), dim3(256), 0, s, outs, gps);
     hipLaunchKernelGGL(k_fold_scatter, dim3(b, nb), dim3(256), 0, s, outs);
     hipLaunchKernelGGL(k_fold_small, dim3(capped_blocks(max_fold_cap[0], 256, per), nb),
                        dim3(256), 0, s, outs, gps);

@@ -54,6 +54,33 @@ def test_column_parallel_qhull_matches_single_call():
         assert v in par[v2t[v], :4]
 
 
+def test_jittered_lattice_columns_with_tiebreak_positions():
+    """A lattice jittered below float32 resolution: the columns agree when the
+    float64 tie-break positions decide the triangulation, the result equals
+    one Qhull call on those positions, and it is a tessellation of the float32
+    particles (every tet non-inverted in float64 tie-break space, every
+    original star closed)."""
+    n_side = 16
+    p64 = synthetic.jittered_lattice64(n_side, 1e-6, seed=3)
+    p = synthetic.jittered_lattice(n_side, 1e-6, seed=3)
+    assert np.array_equal(p, p64.astype(np.float32))
+    g, src = ingest.periodic_ghosts(p, 3.0 / n_side)
+    allp = np.concatenate([p, g]).astype(np.float32)
+    q64 = ingest.ghost_positions64(p, synthetic.lattice_tiebreak(p64, n_side), g, src)
+    # the tie-break positions stay within 1e-2 spacing of the float32 particles
+    assert np.abs(q64 - allp.astype(np.float64)).max() < 0.011 / n_side
+    n = p.shape[0]
+    one = ingest.qhull_tets(allp, q64)
+    par = ingest.tessellate(allp, workers=4, columns=4, min_points=1000, closed_first=n,
+                            qhull_points=q64)
+    assert _stars(one, n) == _stars(par, n)
+    _check_links(par)
+    a, b, c, d = (q64[par[:, k]] for k in range(4))
+    vol = np.einsum("ij,ij->i", b - a, np.cross(c - a, d - a))
+    assert np.all(np.abs(vol) > 0)
+    assert int(ingest.open_star_vertices(par, allp.shape[0])[:n].sum()) == 0
+
+
 def test_decomposed_blocks_layout_and_links():
     pts, bid, counts, bounds = synthetic.plummer_blocks(2, 300, 1500, seed=7)
     w = 0.05
@@ -70,6 +97,13 @@ def test_decomposed_blocks_layout_and_links():
         # every other block's particle within the ghost width is a ghost
         near = (bid != b) & np.all(pts >= lo - np.float32(w), 1) & np.all(pts <= hi + np.float32(w), 1)
         assert gh.shape[0] == int(near.sum())
+        # each ghost's home: the block holding it as an original, and its index there
+        assert hb.ghost_gid.shape == hb.ghost_cell.shape == (gh.shape[0],)
+        for k in range(0, gh.shape[0], 7):
+            home = blocks[int(hb.ghost_gid[k])]
+            assert home.gid != b
+            assert np.array_equal(home.particles[int(hb.ghost_cell[k])], gh[k])
+            assert int(hb.ghost_cell[k]) < home.num_orig
         _check_links(hb.tets)
         v2t = hb.vert_to_tet
         for v in range(0, hb.num_particles, 37):

@@ -64,7 +64,9 @@ enum { TESS2_DENSE_TESS = 0, TESS2_DENSE_CIC = 1 };
 /* grid-point assignment of the TESS estimator */
 enum {
     TESS2_ASSIGN_REFERENCE_SCAN = 0, /* the reference scanline search, bit-exact */
-    TESS2_ASSIGN_EXACT_VORONOI = 1   /* nearest site, ties to the lowest id    */
+    TESS2_ASSIGN_EXACT_VORONOI = 1   /* nearest site, ties to the lowest id
+                                        (several blocks: lowest home key; the
+                                        blocks then need ghost_gid/ghost_cell) */
 };
 
 /* arithmetic of the EXACT_VORONOI assignment */

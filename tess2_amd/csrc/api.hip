@@ -37,10 +37,11 @@ struct DevBlock {
     float *density = nullptr;
     unsigned *count = nullptr;
     uint2 *first = nullptr;
+    uint2 *second = nullptr;
     int *head = nullptr;
     int4 *ovf = nullptr;
     unsigned ovf_cap = 0;
-    uint2 *seg = nullptr;                 // fold segments, 2 * ovf_cap
+    uint2 *seg = nullptr;                 // fold segments, 3 * ovf_cap
     int4 *fold[3] = {nullptr, nullptr, nullptr};
     unsigned fold_cap[3] = {0, 0, 0};
     int *slow = nullptr;
@@ -195,7 +196,7 @@ static int dalloc(tess2_ctx *c, T **p, size_t n)
 static void free_block(DevBlock &b)
 {
     void *ps[] = {b.order,  b.pts,    b.tets,   b.v2t,   b.cc,   b.vol,  b.status,    b.npts,
-                  b.density, b.count, b.first, b.head, b.ovf,  b.slow,
+                  b.density, b.count, b.first, b.second, b.head, b.ovf,  b.slow,
                   b.owner, b.status_ex, b.exflags, b.hstart, b.hfill,
                   b.hsums,  b.hbk,    b.hsorted, b.hkey, b.defq, b.seg, b.fold[0], b.fold[1],
                   b.fold[2]};
@@ -637,8 +638,13 @@ extern "C" int tess2_stage(tess2_ctx *c, const tess2_block_t *blocks, int nblock
 static int alloc_lists(tess2_ctx *c, DevBlock &d, unsigned cap)
 {
     int r = 0;
-    const unsigned c1 = 2 * cap / (FOLD_SMALL_MAX + 1) + 1, c2 = 2 * cap / (FOLD_MID_MAX + 1) + 1;
-    if ((r = dalloc(c, &d.ovf, cap)) || (r = dalloc(c, &d.seg, 2 * (size_t)cap)) ||
+    // segments hold a point's list entries plus its first and second
+    // contributions: at most 3 * cap pairs; the list doubles as the large
+    // fold's scratch of the same size
+    const size_t c3 = 3 * (size_t)cap;
+    const unsigned c1 = (unsigned)(c3 / (FOLD_SMALL_MAX + 1) + 1),
+                   c2 = (unsigned)(c3 / (FOLD_MID_MAX + 1) + 1);
+    if ((r = dalloc(c, &d.ovf, (c3 + 1) / 2)) || (r = dalloc(c, &d.seg, c3)) ||
         (r = dalloc(c, &d.fold[0], cap)) || (r = dalloc(c, &d.fold[1], c1)) ||
         (r = dalloc(c, &d.fold[2], c2)))
         return r;
@@ -656,7 +662,8 @@ static int ensure_grid(tess2_ctx *c, DevBlock &d, size_t n)
     if (n > d.grid_cap) {
         int r = 0;
         if ((r = dalloc(c, &d.density, n)) || (r = dalloc(c, &d.count, n)) ||
-            (r = dalloc(c, &d.first, n)) || (r = dalloc(c, &d.head, n)))
+            (r = dalloc(c, &d.first, n)) || (r = dalloc(c, &d.second, n)) ||
+            (r = dalloc(c, &d.head, n)))
             return r;
         d.grid_cap = n;
     }
@@ -677,6 +684,7 @@ static CellOut make_out(DevBlock &d, bool want_vol)
     o.density = d.density;
     o.count = d.count;
     o.first = d.first;
+    o.second = d.second;
     o.head = d.head;
     o.ovf = d.ovf;
     o.ovf_cap = d.ovf_cap;
@@ -888,7 +896,7 @@ static int enqueue_kernels(tess2_ctx *c, const RunArgs &a)
         mcells = std::max(mcells, d.norig);
         mgrid = std::max<unsigned long long>(mgrid, d.ngrid);
     }
-    launch_fold(c->d_outs, c->d_gps, nb, mcap, mfold, s);
+    launch_fold(c->d_outs, c->d_gps, nb, mcap, mfold, mgrid, s);
     launch_stats(c->d_outs, c->d_gpart, nb, mcells, mgrid, s);
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipMemcpyAsync(c->h_stats, c->d_stats, sizeof(BlockStats) * nb,
@@ -1037,7 +1045,7 @@ extern "C" int tess2_sync(tess2_ctx *c, tess2_stats_t *st)
             st->points_routed += d.hm->ctrs[C_ROUTED];
             st->points_deposited += (long long)d.hm->cstat[5] - d.hm->ctrs[C_OFFBLOCK] -
                                     d.hm->ctrs[C_ROUTED];
-            st->points_shared += d.hm->ctrs[C_OVF];
+            st->points_shared += d.hm->ctrs[C_OVF] + d.hm->ctrs[C_PAIRS];
             st->fp64_fallbacks += d.hm->ctrs[C_FALLBACK] + d.hm->ctrs[C_DEFER];
             st->cells_deferred += d.hm->ctrs[C_SLOW];
             st->tot_mass += d.hm->gsum * (double)d.gp.div;

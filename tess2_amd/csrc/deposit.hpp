@@ -32,6 +32,10 @@ __device__ __forceinline__ void push_point(long long lidx, float m, unsigned key
     if (c == 0u) {
         o.first[lidx] = make_uint2(key, __float_as_uint(m));
         o.density[lidx] = (float)((double)m / (double)div);
+    } else if (c == 1u) {
+        // most shared points have two contributions: the second sits next to
+        // the point too, with no list slot (no second dependent atomic)
+        o.second[lidx] = make_uint2(key, __float_as_uint(m));
     } else {
         const unsigned e = atomicAdd(&o.ctrs[C_OVF], 1u);
         if (e < o.ovf_cap) {

@@ -18,6 +18,8 @@
 //      memory (larger, one 1024-lane workgroup per point).
 // The additions of one point are a chain of dependent float roundings, so
 // each runs on a single lane; only the division by div is done in parallel.
+#include <algorithm>
+
 #include "kernels.hpp"
 
 namespace t2 {
@@ -38,18 +40,45 @@ __device__ __forceinline__ unsigned fold_capacity(const CellOut &o, int cls)
     return cls == 0 ? o.fold_cap[0] : (cls == 1 ? o.fold_cap[1] : o.fold_cap[2]);
 }
 
+// 0. points with exactly two contributions (the common case: a point on the
+// face between two cells) hold both next to the point: added here in the
+// order the sort gives the pair (the second first unless its key is
+// larger); every point with a second contribution is counted
+__global__ __launch_bounds__(256) void k_fold_pairs(const CellOut *__restrict__ outs,
+                                                    const GridP *__restrict__ gps)
+{
+    __shared__ unsigned s_n;
+    const CellOut &o = outs[blockIdx.y];
+    const double dd = (double)gps[blockIdx.y].div;
+    if (threadIdx.x == 0) s_n = 0;
+    __syncthreads();
+    unsigned np = 0;
+    for (unsigned long long g = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;
+         g < o.ngrid; g += (unsigned long long)gridDim.x * blockDim.x) {
+        const unsigned c = o.count[g];
+        if (c < 2u) continue;
+        ++np;
+        if (c != 2u) continue;
+        const uint2 f = o.first[g], q = o.second[g];
+        const bool sw = q.x > f.x;
+        const unsigned m0 = sw ? f.y : q.y, m1 = sw ? q.y : f.y;
+        float rho = (float)(0.0 + (double)__uint_as_float(m0) / dd);
+        rho = (float)((double)rho + (double)__uint_as_float(m1) / dd);
+        o.density[g] = rho;
+    }
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) np += __shfl_xor(np, m);
+    if ((threadIdx.x & 63) == 0 && np) atomicAdd(&s_n, np);
+    __syncthreads();
+    if (threadIdx.x == 0 && s_n) atomicAdd(&o.ctrs[C_PAIRS], s_n);
+}
+
 // 1. one segment per shared point; slot and segment reservations are
 // aggregated per workgroup (one atomic per workgroup and counter per pass)
-// A point with exactly two contributions (the common case: a point on the
-// face between two cells) is folded here at once: its representative entry
-// is its only list entry, and the pair is added in the order the small fold
-// would give it (the list entry first unless its key is larger).
-__global__ __launch_bounds__(256) void k_fold_gather(const CellOut *__restrict__ outs,
-                                                     const GridP *__restrict__ gps)
+__global__ __launch_bounds__(256) void k_fold_gather(const CellOut *__restrict__ outs)
 {
     __shared__ unsigned s_cnt[4][4];   // per wave: points of class 0..2, segment total
     const CellOut &o = outs[blockIdx.y];
-    const double dd = (double)gps[blockIdx.y].div;
     if (fold_overflowed(o)) return;
     const unsigned n = o.ctrs[C_OVF];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -65,23 +94,7 @@ __global__ __launch_bounds__(256) void k_fold_gather(const CellOut *__restrict__
             mine = o.head[lidx] == (int)e;   // the entry that represents its point
             if (mine) {
                 cnt = o.count[lidx];
-                if (cnt == 2u) {
-                    const uint2 f = o.first[lidx];
-                    unsigned k0 = (unsigned)r.y, m0 = (unsigned)r.z, k1 = f.x, m1 = f.y;
-                    if (k0 > k1) {
-                        k0 = f.x;
-                        m0 = f.y;
-                        m1 = (unsigned)r.z;
-                    }
-                    float rho = (float)(0.0 + (double)__uint_as_float(m0) / dd);
-                    rho = (float)((double)rho + (double)__uint_as_float(m1) / dd);
-                    o.density[lidx] = rho;
-                    o.head[lidx] = -(1 + 3);          // class 3: nothing to scatter
-                    mine = false;
-                    cnt = 0;
-                } else {
-                    cls = cnt <= FOLD_SMALL_MAX ? 0 : (cnt <= FOLD_MID_MAX ? 1 : 2);
-                }
+                cls = cnt <= FOLD_SMALL_MAX ? 0 : (cnt <= FOLD_MID_MAX ? 1 : 2);
             }
         }
         const unsigned long long m0 = __ballot(mine && cls == 0);
@@ -117,13 +130,14 @@ __global__ __launch_bounds__(256) void k_fold_gather(const CellOut *__restrict__
             const unsigned long long m = cls == 0 ? m0 : (cls == 1 ? m1 : m2);
             const unsigned idx = s_cnt[w][cls] + (unsigned)__popcll(m & below);
             const unsigned off = s_cnt[w][3] + incl - cnt;
-            if (idx >= fold_capacity(o, cls) || (unsigned long long)off + cnt > 2ull * o.ovf_cap ||
+            if (idx >= fold_capacity(o, cls) || (unsigned long long)off + cnt > 3ull * o.ovf_cap ||
                 idx >= (1u << 28)) {
                 atomicOr(&o.ctrs[C_ERR], 8u);
                 o.head[lidx] = -(1 + 3);              // class 3: skipped by the scatter
             } else {
                 fold_list(o, cls)[idx] = make_int4(lidx, (int)off, (int)cnt, 0);
                 o.seg[off + cnt - 1] = o.first[lidx];
+                o.seg[off + cnt - 2] = o.second[lidx];
                 o.head[lidx] = -(1 + (int)(idx * 4 + cls));
             }
         }
@@ -330,7 +344,7 @@ __global__ __launch_bounds__(1024) void k_fold_large(const CellOut *__restrict__
     unsigned n = o.ctrs[C_FOLD0 + 2];
     if (n > o.fold_cap[2]) n = o.fold_cap[2];
     const double dd = (double)div;
-    uint2 *tmp = reinterpret_cast<uint2 *>(o.ovf);   // 2 * ovf_cap pairs, like seg
+    uint2 *tmp = reinterpret_cast<uint2 *>(o.ovf);   // 3 * ovf_cap pairs, like seg
     const int nt = blockDim.x;
     for (unsigned i = blockIdx.x; i < n; i += gridDim.x) {
         const int4 r = o.fold[2][i];
@@ -397,14 +411,16 @@ static unsigned capped_blocks(unsigned long long work, unsigned per, unsigned ca
 }
 
 void launch_fold(const CellOut *outs, const GridP *gps, int nb, unsigned max_ovf_cap,
-                 const unsigned max_fold_cap[3], hipStream_t s)
+                 const unsigned max_fold_cap[3], unsigned long long max_grid, hipStream_t s)
 {
     if (nb <= 0) return;
     // grids sized by the largest block's capacities, fewer workgroups per
     // block when there are many blocks (the loops stride)
     const unsigned per = nb > 16 ? 256 : 4096;
     const unsigned b = capped_blocks(max_ovf_cap, 256, per);
-    hipLaunchKernelGGL(k_fold_gather, dim3(b, nb), dim3(256), 0, s, outs, gps);
+    const unsigned bg = (unsigned)std::min<unsigned long long>(per, max_grid / 256 + 1);
+    hipLaunchKernelGGL(k_fold_pairs, dim3(bg, nb), dim3(256), 0, s, outs, gps);
+    hipLaunchKernelGGL(k_fold_gather, dim3(b, nb), dim3(256), 0, s, outs);
     hipLaunchKernelGGL(k_fold_scatter, dim3(b, nb), dim3(256), 0, s, outs);
     hipLaunchKernelGGL(k_fold_small, dim3(capped_blocks(max_fold_cap[0], 256, per), nb),
                        dim3(256), 0, s, outs, gps);

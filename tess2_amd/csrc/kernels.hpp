@@ -35,6 +35,7 @@ enum {
     C_ROUTED = 6,     // points deposited into another block's grid
     C_SEG = 7,        // contributions gathered into per-point segments for the fold
     C_FOLD0 = 8,      // shared points per fold class (C_FOLD0 + class, 3 classes)
+    C_PAIRS = 11,     // points with a second contribution
     C_NCTR = 12
 };
 
@@ -62,13 +63,15 @@ struct CellOut {
     float *density;    // [grid]
     unsigned *count;   // [grid] contributions per point
     uint2 *first;      // [grid] first contribution (cell, mass bits)
+    uint2 *second;     // [grid] second contribution (the list holds the third on)
     int *head;         // [grid] one shared-list entry of the point (its representative)
-    int4 *ovf;         // shared-point list: {lidx, key, mass bits, 0}
+    int4 *ovf;         // shared-point list, third contribution on: {lidx, key, mass bits, 0};
+                       // (3 * ovf_cap + 1) / 2 entries, the large fold's scratch
     unsigned ovf_cap;
     unsigned *ctrs;    // [C_NCTR]
     int *slow;         // cells for the large-capacity pass
     unsigned slow_cap;
-    uint2 *seg;        // [2 * ovf_cap] contributions {key, mass bits}, one segment per point
+    uint2 *seg;        // [3 * ovf_cap] contributions {key, mass bits}, one segment per point
     int4 *fold[3];     // shared points per fold class: {lidx, segment start, count, cursor}
     unsigned fold_cap[3];
     BlockStats *st;    // this block's record (st->ctrs == ctrs)
@@ -184,7 +187,7 @@ void launch_cells_ref_batched(const BlockIn *ins, const GridP *gps, const CellOu
 // ordered fold of shared grid points and per-block statistics for nb blocks
 // at once (outs/gps: device arrays; the capacities size the grids)
 void launch_fold(const CellOut *outs, const GridP *gps, int nb, unsigned max_ovf_cap,
-                 const unsigned max_fold_cap[3], hipStream_t s);
+                 const unsigned max_fold_cap[3], unsigned long long max_grid, hipStream_t s);
 // part: nb * 1024 doubles of scratch for the fixed-order grid sums
 void launch_stats(const CellOut *outs, double *part, int nb, int max_cells,
                   unsigned long long max_grid, hipStream_t s);

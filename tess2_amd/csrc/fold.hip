@@ -4,11 +4,13 @@
 // a block's cells in ascending id (IterateCells, src/dense.cpp:220-313),
 // then points received from other blocks (recvd_pts, src/dense.cpp:166-202),
 // each as density = (float)((double)density + mass / div)
-// (src/dense.cpp:290).  The cell kernels record the first contribution of a
-// point next to the point and append later ones to a shared list; this pass
-// replays the additions of every point that got more than one in ascending
-// key order (key = cell id, or the received-point key above 2^31):
-//   1. gather: each shared point (found through its list head) reserves a
+// (src/dense.cpp:290).  The cell kernels record the first and second
+// contribution of a point next to the point and append later ones to a
+// shared list; this pass replays the additions of every point that got more
+// than one in ascending key order (key = cell id, or the received-point key
+// above 2^31):
+//   0. pairs: a grid pass adds the points with exactly two contributions;
+//   1. gather: each larger point (found through its list head) reserves a
 //      contiguous segment, sized by its contribution count, and is filed in
 //      one of three classes by that count;
 //   2. scatter: list entries go to their point's segment;

@@ -70,6 +70,8 @@ def lib():
                                        C.c_float]
         L.o_cell_bounds.argtypes = [C.POINTER(OBlock), C.c_int, F32P, F32P, F32P, F32P,
                                     C.c_int]
+        L.o_scan_mask.argtypes = [I32P, np.ctypeslib.ndpointer(np.uint8, flags="C_CONTIGUOUS"),
+                                  I32P]
         L.o_cell_points.argtypes = [C.POINTER(OBlock), C.POINTER(OArgs), C.c_int, I32P,
                                     F64P, C.c_int, C.POINTER(C.c_int)]
         L.o_iterate_cells.restype = C.c_long
@@ -224,6 +226,20 @@ def distribute_cic(pt, scalar, step, gmin, eps):
     lib().o_distribute_cic(_f3(pt), float(np.float32(scalar)), idx, m, _f3(step),
                            _f3(gmin), float(np.float32(eps)))
     return idx.reshape(8, 3), m
+
+
+def scan_mask(inside):
+    """CellInteriorGridPts' scan (src/dense.cpp:1475-1701) on a box of given
+    PtInCell verdicts, inside[z][y][x] (bool).  Returns (total points,
+    [[(min_xi, max_xi) per y line] per z slice]); min > max = empty line."""
+    m = np.ascontiguousarray(inside, dtype=np.uint8)
+    assert m.ndim == 3
+    nz, ny, nx = m.shape
+    cgp = np.array([nx, ny, nz], np.int32)
+    border = np.zeros(2 * ny * nz, np.int32)
+    tot = lib().o_scan_mask(cgp, m.reshape(-1), border)
+    b = border.reshape(nz, ny, 2)
+    return tot, [[(int(b[z, y, 0]), int(b[z, y, 1])) for y in range(ny)] for z in range(nz)]
 
 
 def cell_bounds(block, cell, cap=1024):

@@ -551,23 +551,17 @@ int o_cell_bounds(const o_block_t *b, int cell, float *cmin, float *cmax,
 
 /* ------------------------------------------------------------------------ */
 /* CellInteriorGridPts(): the scanline search, src/dense.cpp:1475-1701.      */
-/* Returns the per-line runs in border[] and the total point count.          */
+/* Every PtInCell probe of the search is at a grid point cmin_pos + i*step   */
+/* of the cell's box, so the search is a function of the box's verdicts by   */
+/* index; scan_walk runs it on any verdict source and fills the per-line     */
+/* runs border[2*(zi*ny + yi)] = {min_xi, max_xi} (min > max: empty line).   */
 /* ------------------------------------------------------------------------ */
-static int *g_border;
-static long g_border_cap;
+typedef int (*scan_verdict_fn)(const void *ctx, int xi, int yi, int zi);
 
-static int interior_scan(const int *cgp, const float *cmin_pos,
-                         const float *normals, const float *f0, int nf,
-                         const float *step, float eps)
+static int scan_walk(const int *cgp, scan_verdict_fn inside, const void *ctx,
+                     int *border)
 {
-    long need = 2L * cgp[1] * cgp[2];
-    if (need > g_border_cap) {
-        g_border_cap = need + 64;
-        g_border = (int *)realloc(g_border, sizeof(int) * (size_t)g_border_cap);
-    }
-    int *border = g_border;
     int tot = 0;
-    float gp[3];
     int x_left = cgp[0] / 2, x_right = cgp[0] / 2;
     int y_start = 0;
     int x_in_left, x_in_right, min_xi, max_xi;
@@ -575,7 +569,6 @@ static int interior_scan(const int *cgp, const float *cmin_pos,
     int xi, yi, zi, yj, first_x = 0;
 
     for (zi = 0; zi < cgp[2]; zi++) {
-        gp[2] = cmin_pos[2] + (float)zi * step[2];
         border_found = 0;
         z_step_done = 0;
         for (yj = 0; yj < y_start; yj++) {
@@ -583,24 +576,14 @@ static int interior_scan(const int *cgp, const float *cmin_pos,
             border[2 * (zi * cgp[1] + yj) + 1] = 0;
         }
         for (yi = y_start; yi < cgp[1]; yi++) {
-            gp[1] = cmin_pos[1] + (float)yi * step[1];
             /* both stepping flags come from the point at x_left (:1530-1542) */
-            gp[0] = cmin_pos[0] + (float)x_left * step[0];
-            gp[2] = cmin_pos[2] + (float)zi * step[2];
-            if (o_pt_in_cell(gp, normals, f0, nf, eps)) {
-                x_in_left = 1;
-                x_in_right = 1;
-            } else {
-                x_in_left = 0;
-                x_in_right = 0;
-            }
+            x_in_left = x_in_right = inside(ctx, x_left, yi, zi) ? 1 : 0;
             min_xi = cgp[0] - 1;
             max_xi = 0;
 
             /* x step left (:1547-1582) */
             for (xi = x_left; xi >= 0 && xi < cgp[0];) {
-                gp[0] = cmin_pos[0] + (float)xi * step[0];
-                if (o_pt_in_cell(gp, normals, f0, nf, eps)) {
+                if (inside(ctx, xi, yi, zi)) {
                     if (xi < min_xi) min_xi = xi;
                     if (xi > max_xi) max_xi = xi;
                     if (x_in_left) {
@@ -620,8 +603,7 @@ static int interior_scan(const int *cgp, const float *cmin_pos,
             }
             /* x step right (:1585-1620) */
             for (xi = x_right; xi >= 0 && xi < cgp[0];) {
-                gp[0] = cmin_pos[0] + (float)xi * step[0];
-                if (o_pt_in_cell(gp, normals, f0, nf, eps)) {
+                if (inside(ctx, xi, yi, zi)) {
                     if (xi < min_xi) min_xi = xi;
                     if (xi > max_xi) max_xi = xi;
                     if (x_in_right) {
@@ -647,6 +629,8 @@ static int interior_scan(const int *cgp, const float *cmin_pos,
                 if (yi == y_start)
                     first_x = (min_xi + max_xi) / 2;
             }
+            /* first_y is y_start unless this line is the z step's first
+             * with points (:1638-1644) */
             int first_y = y_start;
             if (min_xi <= max_xi && !border_found) {
                 first_y = yi;
@@ -655,15 +639,12 @@ static int interior_scan(const int *cgp, const float *cmin_pos,
             if (min_xi > max_xi && border_found)
                 z_step_done = 1;
 
-            /* loosen y_start for the next z step (:1649-1661) */
+            /* loosen y_start for the next z step: probe down from first_y
+             * at x = first_x in the next slice (:1649-1661) */
             if ((yi == cgp[1] - 1 || z_step_done) && zi + 1 < cgp[2]) {
-                gp[2] = cmin_pos[2] + (float)(zi + 1) * step[2];
-                gp[0] = cmin_pos[0] + (float)first_x * step[0];
-                for (yj = first_y; yj > 0; yj--) {
-                    gp[1] = cmin_pos[1] + (float)yj * step[1];
-                    if (!o_pt_in_cell(gp, normals, f0, nf, eps))
+                for (yj = first_y; yj > 0; yj--)
+                    if (!inside(ctx, first_x, yj, zi + 1))
                         break;
-                }
                 y_start = yj;
             }
             if (z_step_done) {
@@ -676,6 +657,57 @@ static int interior_scan(const int *cgp, const float *cmin_pos,
         }
     }
     return tot;
+}
+
+/* PtInCell verdicts of a cell's box (the probes of :1525-1657) */
+typedef struct {
+    const float *cmin_pos, *step, *normals, *f0;
+    int nf;
+    float eps;
+} cell_box_t;
+
+static int cell_box_inside(const void *ctx, int xi, int yi, int zi)
+{
+    const cell_box_t *c = (const cell_box_t *)ctx;
+    float gp[3];
+    gp[0] = c->cmin_pos[0] + (float)xi * c->step[0];
+    gp[1] = c->cmin_pos[1] + (float)yi * c->step[1];
+    gp[2] = c->cmin_pos[2] + (float)zi * c->step[2];
+    return o_pt_in_cell(gp, c->normals, c->f0, c->nf, c->eps);
+}
+
+/* verdicts given as a mask, x fastest */
+typedef struct {
+    const unsigned char *mask;
+    int nx, ny;
+} mask_box_t;
+
+static int mask_box_inside(const void *ctx, int xi, int yi, int zi)
+{
+    const mask_box_t *m = (const mask_box_t *)ctx;
+    return m->mask[((long)zi * m->ny + yi) * m->nx + xi] != 0;
+}
+
+static int *g_border;
+static long g_border_cap;
+
+static int interior_scan(const int *cgp, const float *cmin_pos,
+                         const float *normals, const float *f0, int nf,
+                         const float *step, float eps)
+{
+    long need = 2L * cgp[1] * cgp[2];
+    if (need > g_border_cap) {
+        g_border_cap = need + 64;
+        g_border = (int *)realloc(g_border, sizeof(int) * (size_t)g_border_cap);
+    }
+    cell_box_t c = {cmin_pos, step, normals, f0, nf, eps};
+    return scan_walk(cgp, cell_box_inside, &c, g_border);
+}
+
+int o_scan_mask(const int *cgp, const unsigned char *inside, int *border)
+{
+    mask_box_t m = {inside, cgp[0], cgp[1]};
+    return scan_walk(cgp, mask_box_inside, &m, border);
 }
 
 /* ------------------------------------------------------------------------ */

@@ -106,6 +106,13 @@ void o_distribute_cic(const float *pt, float scalar, int *idx24, float *mass8,
 int o_cell_bounds(const o_block_t *b, int cell, float *bmin, float *bmax,
                   float *normals, float *f0, int cap);
 
+/* The CellInteriorGridPts scan state machine (src/dense.cpp:1475-1701) on a
+ * box of cgp[0] x cgp[1] x cgp[2] points whose PtInCell verdicts are given
+ * as a mask (1 = inside, x fastest).  Fills border[2*(zi*cgp[1] + yi)] =
+ * {min_xi, max_xi} for every line (min > max: no points) and returns the
+ * total point count.  o_cell_points runs the same walk on PtInCell. */
+int o_scan_mask(const int *cgp, const unsigned char *inside, int *border);
+
 /* Grid points of one cell (IterateCells body up to the deposit,
  * src/dense.cpp:250-268 + CellGridPts :1363-1458).  Writes global grid
  * indices (3 ints each) and per-point masses (double, as grid_pt_t::mass).

@@ -134,9 +134,10 @@ def cpu_baseline_blocks(blocks, grid, target_s):
             "sampled": sampled}
 
 
-def cpu_baseline(hb, grid, target_s):
+def cpu_baseline(hb, grid, target_s, full_max_s=0.0):
     """The oracle (scalar C restatement of the reference path) on the first M
-    cells of the same block, scaled to the whole block."""
+    cells of the same block, scaled to the whole block; on every cell when
+    the whole block is estimated to take at most full_max_s."""
     from oracle import oracle as O
     O.build()
     ob_full = O.Block.from_host(hb)
@@ -152,7 +153,7 @@ def cpu_baseline(hb, grid, target_s):
         vol = O.volumes(sub, cc_all)
         d = O.dense_single(sub, grid)
         t2 = time.perf_counter()
-        timed.result = (vol, d["status"], d["npts"])
+        timed.result = (vol, d["status"], d["npts"], d["density"] if m == n else None)
         return (t1 - t0) * T / mt + (t2 - t1) * n / m, t2 - t0
 
     cc_all = O.circumcenters(hb.tets, hb.particles)   # for volume(), untimed: the
@@ -161,50 +162,66 @@ def cpu_baseline(hb, grid, target_s):
     est, spent = timed(m)
     per_cell = spent / m
     m2 = int(min(n, max(m, target_s / max(per_cell, 1e-9))))
+    if full_max_s and est <= full_max_s:
+        m2 = n                       # the whole block: full-grid density check
     if m2 > m:
         est, spent = timed(m2)
         m = m2
     del ob_full
-    G = int(np.prod(grid))
+    what = (f"all {n} cells" if m == n else f"first {m} of {n} cells")
     return {"cores": 1, "kind": "port",
-            "sample": f"first {m} of {n} cells ({spent:.1f}s of CPU work: circumcenters of "
+            "sample": f"{what} ({spent:.1f}s of CPU work: circumcenters of "
                       f"the same fraction of tets, volume() and IterateCells over those "
-                      f"cells), scaled to all cells; single thread",
+                      f"cells){'' if m == n else ', scaled to all cells'}; single thread",
             "est_full_s": est, "sampled": timed.result}
 
 
 def exact_paths(run, grid, G, args, ref_density):
     """The exact-Voronoi assignment in the mixed FP32/FP64 path and in the
-    all-FP64 path on the same staged block: timing, FP64 fallbacks, the
-    agreement of the two (owner of every grid point, densities), and how the
-    reference scan's point sets differ from the exact assignment (grid points
-    the scan leaves without mass that the exact assignment gives to a cell)."""
+    all-FP64 path on the same staged block: timing of the full step
+    (circumcenters, volumes, assignment, deposit) and of the assignment and
+    deposit alone, FP64 fallbacks, the agreement of the two (owner of every
+    grid point, densities), and how the reference scan's point sets differ
+    from the exact assignment (grid points the scan leaves without mass that
+    the exact assignment gives to a cell).  Returns (report, {precision:
+    (owner, density)})."""
     from tess2_amd import host
     res = {}
     outs = {}
-    for name, prec in (("mixed", host.MIXED), ("fp64", host.FP64)):
+
+    def timed(prec, vols):
         for _ in range(args.warmup):
-            run.run(grid, assignment=host.EXACT_VORONOI, precision=prec, want_volumes=False)
+            run.run(grid, assignment=host.EXACT_VORONOI, precision=prec, want_volumes=vols)
             run.sync()
         run.kernel_time()
         t0 = time.perf_counter()
         for _ in range(args.steps):
-            run.run(grid, assignment=host.EXACT_VORONOI, precision=prec, want_volumes=False)
+            run.run(grid, assignment=host.EXACT_VORONOI, precision=prec, want_volumes=vols)
             st = run.sync()
         ms = (time.perf_counter() - t0) * 1e3 / args.steps
         kms, _ = run.kernel_time()
+        return ms, kms, st
+
+    for name, prec in (("mixed", host.MIXED), ("fp64", host.FP64)):
+        ms_a, _, _ = timed(prec, False)
+        ms, kms, st = timed(prec, True)
         outs[name] = (run.owner(0), run.fetch(0)["density"])
         res[name] = {"ms_per_step": ms, "grid_pts_per_s": G / (ms * 1e-3),
+                     "assignment_only_ms_per_step": ms_a,
                      "owner_kernel_ms": kms, "fp64_fallbacks": int(st["fp64_fallbacks"]),
                      "fp32_filter_failure_rate": float(st["fp64_fallbacks"]) / G,
                      "cells_deposited": int(st["cells_scan"] + st["cells_cic"])}
-        log(f"[exact {name}] {ms:.3f} ms/step, owner kernel {kms:.3f} ms, "
-            f"fallbacks {st['fp64_fallbacks']}")
+        log(f"[exact {name}] {ms:.3f} ms/step (assignment and deposit alone {ms_a:.3f}), "
+            f"owner kernel {kms:.3f} ms, fallbacks {st['fp64_fallbacks']}")
+    res["step"] = ("ms_per_step: circumcenters, volumes, exact assignment, deposit; "
+                   "assignment_only_ms_per_step: the same without the volume pass")
     om, dm = outs["mixed"]
     of, df = outs["fp64"]
     nz = df != 0
     rel = np.abs(dm[nz].astype(np.float64) - df[nz]) / np.abs(df[nz].astype(np.float64))
     res["speedup_mixed_vs_fp64"] = res["fp64"]["ms_per_step"] / res["mixed"]["ms_per_step"]
+    res["speedup_mixed_vs_fp64_assignment_only"] = (res["fp64"]["assignment_only_ms_per_step"] /
+                                                    res["mixed"]["assignment_only_ms_per_step"])
     res["owner_identical"] = bool(np.array_equal(om, of))
     res["max_rel_density_diff"] = float(rel.max()) if rel.size else 0.0
     res["rms_rel_density_diff"] = float(np.sqrt(np.mean(rel ** 2))) if rel.size else 0.0
@@ -213,6 +230,25 @@ def exact_paths(run, grid, G, args, ref_density):
                             "points_massless_exact": int(ex0.sum()),
                             "points_missed_by_scan": int((scan0 & ~ex0).sum()),
                             "grid_points": int(G)}
+    return res, outs
+
+
+def fp64_errors(hb, grid, gpu_density, exact_outs):
+    """max and RMS relative error of the GPU's float32 densities against the
+    FP64 oracle: the reference scan's density against the FP64 accumulation
+    of the same point sets, and each exact-assignment precision's density
+    against the FP64 exact oracle (whose owners the GPU's must equal)."""
+    from oracle import oracle as O
+    ob = O.Block.from_host(hb)
+    t0 = time.perf_counter()
+    res = {"reference_scan": O.rel_error(gpu_density, O.dense_single_fp64(ob, grid))}
+    if exact_outs is not None:
+        want = O.exact_dense(ob, grid)
+        for name, (own, den) in exact_outs.items():
+            e = O.rel_error(den, want["density"])
+            e["owner_identical"] = bool(np.array_equal(own, want["owner"]))
+            res["exact_" + name] = e
+    res["oracle_s"] = time.perf_counter() - t0
     return res
 
 
@@ -223,10 +259,11 @@ def _vol_bits(a):
     return u
 
 
-def sample_parity(sampled, gpu_cells, gpu_vol, multi):
+def sample_parity(sampled, gpu_cells, gpu_vol, gpu_density, multi):
     """The GPU's per-cell statuses and point counts (and volumes, one block)
     of the full-size run against the oracle's on the cells the CPU baseline
-    processed."""
+    processed; when it processed every cell, the density of every grid point
+    too."""
     if multi:
         ok, cells = True, 0
         for gid, status, npts in sampled:
@@ -234,13 +271,18 @@ def sample_parity(sampled, gpu_cells, gpu_vol, multi):
             ok = ok and np.array_equal(gs, status) and np.array_equal(gn, npts)
             cells += len(status)
         return {"cells": cells, "status_npts_identical": bool(ok)}
-    vol, status, npts = sampled
+    vol, status, npts, density = sampled
     m = len(status)
     gs, gn = next(iter(gpu_cells.values()))
-    return {"cells": m,
-            "status_npts_identical": bool(np.array_equal(gs[:m], status) and
-                                          np.array_equal(gn[:m], npts)),
-            "volumes_identical": bool(np.array_equal(_vol_bits(gpu_vol[:m]), _vol_bits(vol)))}
+    out = {"cells": m,
+           "status_npts_identical": bool(np.array_equal(gs[:m], status) and
+                                         np.array_equal(gn[:m], npts)),
+           "volumes_identical": bool(np.array_equal(_vol_bits(gpu_vol[:m]), _vol_bits(vol)))}
+    if density is not None:
+        # every cell ran: the whole grid, bit for bit
+        out["density_identical"] = bool(np.array_equal(_vol_bits(gpu_density), _vol_bits(density)))
+        out["grid_points"] = int(density.size)
+    return out
 
 
 def main():
@@ -253,6 +295,10 @@ def main():
     ap.add_argument("--no-exact", action="store_true",
                     help="skip the exact-Voronoi mixed / FP64 measurement")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cpu-full-max", type=float, default=45.0,
+                    help="run the oracle over the whole block (full-grid density check "
+                         "and FP64 error report) when it is estimated to take at most this "
+                         "many seconds; 0 = always sample")
     ap.add_argument("--no-graph", action="store_true",
                     help="time eager launches instead of graph replays (for tracing tools)")
     ap.add_argument("--save-block", default=None,
@@ -358,24 +404,29 @@ def main():
                      "kernel": kernel, "kernel_ms": kern_ms, "alg_bytes": b_alg},
     }
     run.set_option("graph", 0)
-    # per-cell results of the reference scan, for the check against the
-    # oracle's sample below (the exact runs overwrite them)
+    # per-cell results and density of the reference scan, for the checks
+    # against the oracle below (the exact runs overwrite them)
     gpu_cells = {b.gid: run.cells(i) for i, b in enumerate(run.blocks)}
     gpu_vol = run.fetch(0, ngrid=False, volumes=True)["volumes"] if not multi else None
+    gpu_density = run.fetch(0)["density"] if not multi else None
+    exact_outs = None
     if not args.no_exact and not multi:
-        ref_density = run.fetch(0)["density"]
-        out["exact_voronoi"] = exact_paths(run, grid, G, args, ref_density)
+        out["exact_voronoi"], exact_outs = exact_paths(run, grid, G, args, gpu_density)
     if not args.no_cpu:
         log("[cpu] timing the oracle on a sample")
         if multi:
             cb = cpu_baseline_blocks(blocks, grid, args.cpu_seconds)
         else:
-            cb = cpu_baseline(blocks[0], grid, args.cpu_seconds)
+            cb = cpu_baseline(blocks[0], grid, args.cpu_seconds, args.cpu_full_max)
         cb["value"] = G / cb["est_full_s"]
         cb["unit"] = "grid pts/s"
-        out["sample_parity"] = sample_parity(cb.pop("sampled"), gpu_cells, gpu_vol, multi)
+        sampled = cb.pop("sampled")
+        out["sample_parity"] = sample_parity(sampled, gpu_cells, gpu_vol, gpu_density, multi)
         log(f"[cpu] sample parity: {out['sample_parity']}")
         out["cpu_baseline"] = cb
+        if not multi and sampled[3] is not None:
+            out["fp64_error"] = fp64_errors(blocks[0], grid, gpu_density, exact_outs)
+            log(f"[cpu] errors against the FP64 oracle: {out['fp64_error']}")
     print(json.dumps(out), flush=True)
 
 

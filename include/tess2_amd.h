@@ -183,7 +183,9 @@ int tess2_run(tess2_ctx *ctx, int alg, int num_given_bounds, const float *given_
 /* Wait for the last run; fill stats (device_ms = event time of the run). */
 int tess2_sync(tess2_ctx *ctx, tess2_stats_t *stats);
 
-/* Copy results of block b to the host (NULL pointers are skipped). */
+/* Copy results of block b to the host (NULL pointers are skipped).
+ * Volumes exist only after a TESS run with want_volumes (or tess2_volumes);
+ * asking for them after any other run returns TESS2_E_ARG. */
 int tess2_fetch(tess2_ctx *ctx, int b, float *density, float *volumes,
                 float *circumcenters);
 

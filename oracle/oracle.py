@@ -77,6 +77,9 @@ def lib():
         L.o_iterate_cells.restype = C.c_long
         L.o_iterate_cells.argtypes = [C.POINTER(OBlock), C.POINTER(OArgs), I32P, I32P,
                                       F32P, vp, vp, vp, vp, vp, C.c_long, vp]
+        L.o_iterate_cells_fp64.restype = C.c_long
+        L.o_iterate_cells_fp64.argtypes = [C.POINTER(OBlock), C.POINTER(OArgs), I32P, I32P,
+                                           F64P]
         L.o_iterate_cells_cic.restype = C.c_long
         L.o_iterate_cells_cic.argtypes = [C.POINTER(OBlock), C.POINTER(OArgs), I32P, I32P,
                                           F32P, vp, vp, vp, C.c_long, vp]
@@ -325,6 +328,33 @@ def dense_single(block, gnum, mass=1.0, eps=1e-4, cic=False, project=False,
     return dict(density=den, status=status, npts=npts, offblock=off, tot_mass=tm,
                 grid_phys_mins=gmin, grid_phys_maxs=gmax, grid_step_size=step,
                 block_min_idx=bmn, block_max_idx=bmx, block_num_idx=bnum, args=args)
+
+
+def dense_single_fp64(block, gnum, mass=1.0, eps=1e-4):
+    """The reference scan's point sets of dense_single with every mass and
+    density in float64 (o_iterate_cells_fp64): the FP64 density against
+    which the float32 density's rounding error is reported."""
+    dmin = np.array(block.cs.bounds_min[:], np.float32)
+    dmax = np.array(block.cs.bounds_max[:], np.float32)
+    gmin, gmax, step = grid_step_params(dmin, dmax, gnum)
+    bmn, bmx, bnum = block_grid_params(dmin, dmax, gmin, step, eps, dmin, dmax, gnum)
+    args = make_args(mass, dmin, dmax, gmin, step, eps, gnum)
+    den = np.zeros(int(bnum[0]) * int(bnum[1]) * int(bnum[2]), np.float64)
+    lib().o_iterate_cells_fp64(C.byref(block.cs), C.byref(args), bmn, bnum, den)
+    return den
+
+
+def rel_error(f32, f64):
+    """max and RMS relative error of a float32 density against float64,
+    over the points where the float64 density is nonzero; and whether both
+    are zero at the same points."""
+    a = np.asarray(f32, np.float64).ravel()
+    b = np.asarray(f64, np.float64).ravel()
+    nz = b != 0
+    rel = np.abs(a[nz] - b[nz]) / np.abs(b[nz])
+    return {"max_rel": float(rel.max()) if rel.size else 0.0,
+            "rms_rel": float(np.sqrt(np.mean(rel * rel))) if rel.size else 0.0,
+            "points": int(nz.sum()), "zeros_identical": bool(np.array_equal(a == 0, ~nz))}
 
 
 def exact_dense(block, gnum, mass=1.0, eps=1e-4):

@@ -812,9 +812,8 @@ int o_cell_points(const o_block_t *b, const o_args_t *a, int cell, int *idx3,
  * block bounds whose local index falls outside the block grid (possible on a
  * shared face, where BlockGridParams gives the index to the upper block; the
  * reference would index out of its row there) is treated as off-block. */
-static int deposit(const o_block_t *b, const o_args_t *a, const int *bmin_idx,
-                   const int *bnum_idx, float *density, const int *gidx,
-                   double m, float div, double *tot_mass)
+static int in_block_index(const o_block_t *b, const o_args_t *a, const int *bmin_idx,
+                          const int *bnum_idx, const int *gidx)
 {
     float pos[3];
     o_idx2phys(gidx, pos, a->grid_step_size, a->grid_phys_mins);
@@ -823,10 +822,17 @@ static int deposit(const o_block_t *b, const o_args_t *a, const int *bmin_idx,
                  (a->project || (lo[2] >= 0 && lo[2] < bnum_idx[2]));
     if (ongrid && pos[0] >= b->bounds_min[0] && pos[0] <= b->bounds_max[0] &&
         pos[1] >= b->bounds_min[1] && pos[1] <= b->bounds_max[1] &&
-        pos[2] >= b->bounds_min[2] && pos[2] <= b->bounds_max[2]) {
-        int l[3] = {gidx[0] - bmin_idx[0], gidx[1] - bmin_idx[1],
-                    gidx[2] - bmin_idx[2]};
-        int idx = o_index(l, bnum_idx, a->project, a->proj_plane);
+        pos[2] >= b->bounds_min[2] && pos[2] <= b->bounds_max[2])
+        return o_index(lo, bnum_idx, a->project, a->proj_plane);
+    return -1;
+}
+
+static int deposit(const o_block_t *b, const o_args_t *a, const int *bmin_idx,
+                   const int *bnum_idx, float *density, const int *gidx,
+                   double m, float div, double *tot_mass)
+{
+    int idx = in_block_index(b, a, bmin_idx, bnum_idx, gidx);
+    if (idx >= 0) {
         density[idx] = (float)((double)density[idx] + m / (double)div);
         if (tot_mass)
             *tot_mass += m;
@@ -878,6 +884,44 @@ long o_iterate_cells(const o_block_t *b, const o_args_t *a,
                 }
                 noff++;
             }
+        }
+    }
+    return noff;
+}
+
+long o_iterate_cells_fp64(const o_block_t *b, const o_args_t *a,
+                          const int *block_min_idx, const int *block_num_idx,
+                          double *density64)
+{
+    const double div = a->project
+                           ? (double)a->grid_step_size[0] * (double)a->grid_step_size[1]
+                           : (double)a->grid_step_size[0] * (double)a->grid_step_size[1] *
+                                 (double)a->grid_step_size[2];
+    long noff = 0;
+    ensure_faces(64);
+    if (!g_pcap) {
+        g_pcap = 4096;
+        g_pidx = (int *)malloc(sizeof(int) * 3 * (size_t)g_pcap);
+        g_pmass = (double *)malloc(sizeof(double) * (size_t)g_pcap);
+    }
+    for (int cell = 0; cell < b->num_orig_particles; cell++) {
+        int st, n;
+        while ((n = o_cell_points(b, a, cell, g_pidx, g_pmass, g_pcap, &st)) < 0) {
+            g_pcap *= 4;
+            g_pidx = (int *)realloc(g_pidx, sizeof(int) * 3 * (size_t)g_pcap);
+            g_pmass = (double *)realloc(g_pmass, sizeof(double) * (size_t)g_pcap);
+        }
+        /* the reference's mass / (float)N (:1692) taken in double; CIC
+         * corners keep the reference's float weights */
+        const double mscan = (double)a->mass / (double)n;
+        for (int i = 0; i < n; i++) {
+            int idx = in_block_index(b, a, block_min_idx, block_num_idx, &g_pidx[3 * i]);
+            if (idx < 0) {
+                noff++;
+                continue;
+            }
+            const double m = st == O_CELL_SCAN ? mscan : g_pmass[i];
+            density64[idx] += m / div;
         }
     }
     return noff;

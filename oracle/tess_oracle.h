@@ -132,6 +132,16 @@ long o_iterate_cells(const o_block_t *b, const o_args_t *a,
                      int *off_idx3, double *off_mass, int *off_cell,
                      long off_cap, double *tot_mass);
 
+/* FP64 counterpart of o_iterate_cells for error reports: the same cells,
+ * point sets and in-block rule, but each scan point's mass is mass/N in
+ * double (the reference divides in float, :1692; CIC corners keep the
+ * reference's float weights) and density64 (zeroed by the caller)
+ * accumulates m/div in double, div the product of the steps in double.
+ * Returns the number of off-block points skipped. */
+long o_iterate_cells_fp64(const o_block_t *b, const o_args_t *a,
+                          const int *block_min_idx, const int *block_num_idx,
+                          double *density64);
+
 /* IterateCellsCic (src/dense.cpp:486-562) for one block, same conventions. */
 long o_iterate_cells_cic(const o_block_t *b, const o_args_t *a,
                          const int *block_min_idx, const int *block_num_idx,

@@ -30,6 +30,7 @@ struct DevBlock {
     int *order = nullptr;                    // cells in Morton order
     std::vector<int> tet_new;                // caller's tet id -> device tet id
     float *vol = nullptr;
+    bool vol_valid = false;                  // the last run wrote vol
     int *status = nullptr, *npts = nullptr;
     // grid
     GridP gp{};
@@ -908,6 +909,9 @@ static int enqueue_run(tess2_ctx *c, const RunArgs &a)
 {
     int r = prepare_run(c, a);
     if (r) return r;
+    // the cell kernels write volume() only in a TESS run that asks for it
+    for (DevBlock &d : c->blocks)
+        d.vol_valid = a.alg == TESS2_DENSE_TESS && a.want_vol != 0;
     hipStream_t s = c->stream;
     if (c->graph_on) {
         if (!c->gexec || c->ggen != c->gen || !same_args(a, c->gargs)) {
@@ -1179,8 +1183,11 @@ extern "C" int tess2_fetch(tess2_ctx *c, int b, float *density, float *volumes,
     DevBlock &d = c->blocks[b];
     if (density && d.density)
         HIPCHK(c, hipMemcpy(density, d.density, sizeof(float) * d.ngrid, hipMemcpyDeviceToHost));
-    if (volumes && d.norig)
+    if (volumes && d.norig) {
+        if (!d.vol_valid)
+            return fail(c, TESS2_E_ARG, "block %d: the last run computed no volumes", b);
         HIPCHK(c, hipMemcpy(volumes, d.vol, sizeof(float) * d.norig, hipMemcpyDeviceToHost));
+    }
     if (circumcenters && d.nt) {
         std::vector<float4> tmp(d.nt);
         HIPCHK(c, hipMemcpy(tmp.data(), d.cc, sizeof(float4) * d.nt, hipMemcpyDeviceToHost));
@@ -1278,6 +1285,7 @@ extern "C" int tess2_volumes(tess2_ctx *c, const tess2_block_t *block, float *ou
     }
     if (d.hm->ctrs[C_ERR] & 4u)
         return fail(c, TESS2_E_CAPACITY, "a cell exceeds the per-cell capacities");
+    d.vol_valid = true;
     return tess2_fetch(c, 0, nullptr, out, nullptr);
 }
 

@@ -73,6 +73,40 @@ def test_large_cells_batched(oracle):
         assert np.array_equal(_bits(res.density[i]), _bits(want["density"][i]))
 
 
+def _blob_points(n_blob, n_bg, seed):
+    """A tight Gaussian blob in a uniform background."""
+    rng = np.random.default_rng(seed)
+    blob = 0.5 + 0.004 * rng.standard_normal((n_blob, 3))
+    return np.concatenate([blob, rng.random((n_bg, 3))]).astype(np.float32)
+
+
+@pytest.mark.parametrize("graph", [0, 1])
+def test_shared_list_overflow_rerun(oracle, graph):
+    """~72k blob cells on an 8^3 grid: each blob cell's box holds one grid
+    point, (3, 3, 3), and a one-point-wide scan line always keeps its point
+    (src/dense.cpp:1543-1544: min_xi = nx - 1 = 0 <= max_xi = 0), so every
+    blob cell deposits there.  ~72k contributions beyond the second overflow
+    the shared-point list (65536 entries at first); the run is redone with
+    larger lists (in graph mode the graph is recorded again) and must come
+    out bit-exact, with one segmented fold of ~72k contributions."""
+    say("list overflow, graph", graph)
+    hb = ingest.make_block(_blob_points(72000, 3000, 11), 75000, (0, 0, 0), (1, 1, 1))
+    g = (8, 8, 8)
+    want = oracle.dense_single(oracle.Block.from_host(hb), g)
+    run = D.StagedRun([hb])
+    run.set_option("graph", graph)
+    for rep in range(2):
+        run.run(g, want_volumes=False)
+        st = run.sync()
+        say("  run", rep, "points_shared", st["points_shared"])
+        assert st["points_shared"] > 65536
+        den = run.fetch(0, ngrid=512)["density"]
+        s, npts = run.cells(0)
+        assert np.array_equal(s, want["status"])
+        assert np.array_equal(npts, want["npts"])
+        assert np.array_equal(_bits(den), _bits(want["density"]))
+
+
 def test_cell_beyond_all_capacities_is_an_error():
     say("hub 200")
     hb = ingest.periodic_block(_hub_points(200, 7), 0.2)   # 200 faces > 128

@@ -27,6 +27,8 @@ def say(*a):
 def blocks():
     out = {}
     out["grf"] = ingest.periodic_block(synthetic.grf_lattice(24, 0.3, seed=5), 4.0 / 24)
+    # rms 0.5 spacing, 2 grid points per spacing, as the 16.8M-particle workload
+    out["grf05"] = ingest.periodic_block(synthetic.grf_lattice(24, 0.5, seed=3), 4.0 / 24)
     out["uniform"] = ingest.periodic_block(synthetic.uniform(3000, seed=41), 4.0 / 14)
     # exact dyadic lattice (sites at odd multiples of 1/16) on a 17^3 grid
     # (points at multiples of 1/16): many grid points exactly equidistant
@@ -62,7 +64,7 @@ def _check(oracle, hb, g, prec):
 
 
 @pytest.mark.parametrize("prec", [D.MIXED, D.FP64])
-@pytest.mark.parametrize("name,g", [("grf", 40), ("uniform", 32), ("lattice", 17),
+@pytest.mark.parametrize("name,g", [("grf", 40), ("grf05", 48), ("uniform", 32), ("lattice", 17),
                                     ("jitter", 81)])
 def test_exact_assignment(oracle, blocks, name, g, prec):
     say("exact", name, g, "mixed" if prec == D.MIXED else "fp64")

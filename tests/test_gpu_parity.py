@@ -48,6 +48,11 @@ def blocks():
     p = rng.random((4096, 3)).astype(np.float32)
     out["p2"] = ingest.make_block(p, 4096, (0, 0, 0), (1, 1, 1))
     out["grf"] = ingest.periodic_block(synthetic.grf_lattice(24, 0.3, seed=5), 4.0 / 24)
+    # the 16.8M-particle workload's field at small size: rms 0.5 spacing,
+    # 2 grid points per lattice spacing (24^3 sites -> 48^3 grid)
+    out["grf05"] = ingest.periodic_block(synthetic.grf_lattice(24, 0.5, seed=3), 4.0 / 24)
+    # the 32,768-particle uniform workload itself (seed 0, ghosts 4/32 wide)
+    out["uniform32k"] = ingest.periodic_block(synthetic.config_particles(0), 4.0 / 32)
     # exact lattice: Qhull's Qt output has flat tets with NaN circumcenters
     out["lattice"] = ingest.periodic_block(synthetic.jittered_lattice(8, 0.0, seed=4), 3.0 / 8)
     # lattice + 1e-6-spacing jitter, mostly rounded away in float32; the
@@ -59,7 +64,10 @@ def blocks():
     return out
 
 
-@pytest.mark.parametrize("name", ["periodic4k", "p2", "grf", "lattice", "jitter"])
+ALL = ["periodic4k", "p2", "grf", "grf05", "uniform32k", "lattice", "jitter"]
+
+
+@pytest.mark.parametrize("name", ALL)
 def test_circumcenters_bitexact(oracle, blocks, name):
     hb = blocks[name]
     say("circumcenters", name, hb.num_tets)
@@ -68,7 +76,7 @@ def test_circumcenters_bitexact(oracle, blocks, name):
     assert bits_equal(got, want), first_diff(got, want)
 
 
-@pytest.mark.parametrize("name", ["periodic4k", "p2", "grf", "lattice", "jitter"])
+@pytest.mark.parametrize("name", ALL)
 def test_volumes_bitexact(oracle, blocks, name):
     hb = blocks[name]
     say("volumes", name, hb.num_orig)
@@ -83,7 +91,8 @@ def _oracle_dense(oracle, hb, g, cic=False, mass=1.0):
 
 
 @pytest.mark.parametrize("name,g", [("periodic4k", 32), ("periodic4k", 64), ("p2", 64),
-                                    ("periodic4k", 8), ("grf", 40), ("lattice", 17),
+                                    ("periodic4k", 8), ("grf", 40), ("grf05", 48),
+                                    ("uniform32k", 64), ("lattice", 17),
                                     ("lattice", 33), ("jitter", 80)])
 def test_dense_reference_scan_bitexact(oracle, blocks, name, g):
     hb = blocks[name]
@@ -122,6 +131,24 @@ def test_staged_runs_repeat_identically(oracle, blocks):
     for o in outs:
         assert bits_equal(o, want["density"])
     assert st["cells_scan"] + st["cells_cic"] == int((want["status"] <= 1).sum())
+
+
+def test_fetch_volumes_needs_a_volume_run(blocks):
+    """Volumes exist only after a TESS run that computed them."""
+    say("fetch volumes")
+    run = D.StagedRun([blocks["periodic4k"]])
+    run.run((16, 16, 16), want_volumes=False)
+    run.sync()
+    with pytest.raises(D.Tess2Error) as e:
+        run.fetch(0, volumes=True)
+    assert "no volumes" in str(e.value)
+    run.run((16, 16, 16), alg=D.DENSE_CIC)
+    run.sync()
+    with pytest.raises(D.Tess2Error):
+        run.fetch(0, volumes=True)
+    run.run((16, 16, 16))
+    run.sync()
+    assert (run.fetch(0, volumes=True)["volumes"] > 0).any()
 
 
 def test_tiny_and_empty_blocks(oracle):

@@ -272,3 +272,24 @@ def test_multiblock_model_conserves_mass(oracle):
     for mn, mx, nm in r["grids"]:
         cover[mn[2]:mx[2] + 1, mn[1]:mx[1] + 1, mn[0]:mx[0] + 1] += 1
     assert (cover == 1).all()
+
+
+def test_fp64_density_of_the_scan(oracle):
+    """The FP64 accumulation of the scan's point sets: mass conservation in
+    closed form (sum rho * h^3 = mass x deposited cells, to double rounding)
+    and the float32 density within float rounding of it, zero at the same
+    points."""
+    p = synthetic.grf_lattice(16, 0.3, seed=8)
+    hb = ingest.periodic_block(p, 4.0 / 16)
+    ob = oracle.Block.from_host(hb)
+    g = (32, 32, 32)
+    r = oracle.dense_single(ob, g, mass=1.5)
+    d64 = oracle.dense_single_fp64(ob, g, mass=1.5)
+    ncell = int(((r["status"] == 0) | (r["status"] == 1)).sum())
+    h3 = np.prod(r["grid_step_size"].astype(np.float64))
+    assert len(r["offblock"][1]) == 0
+    assert abs(d64.sum() * h3 - 1.5 * ncell) <= 1e-12 * 1.5 * ncell
+    e = oracle.rel_error(r["density"], d64)
+    assert e["zeros_identical"] and e["points"] > 0
+    # each point is a chain of float(double + double) roundings: a few ulp
+    assert 0 < e["max_rel"] < 1e-6 and e["rms_rel"] < 1e-7

@@ -12,7 +12,8 @@ calls the same walk with PtInCell as the verdict).
 
 The mutation tests plant plausible mistakes in a copy of the oracle source
 (compiled into a temporary directory, on the CPU) and check that these known
-answers reject each one: x_in_right taken at x_right, x_left moved to 0 when
+answers reject each one: x_in_right taken at x_right, a line's run starting
+empty (min > max) instead of at (nx - 1, 0), x_left moved to 0 when
 the left walk runs off the box, first_x taken from the z step's first found
 line instead of the y_start line, no early z-step exit, the y_start probe at
 x_left instead of first_x, the BFS visiting a tet's slots in reverse, an edge
@@ -116,6 +117,14 @@ SCAN_CASES = [
                      (1, 0): [0, 1, 2], (1, 1): [1, 2], (1, 2): [1],
                      (2, 0): [1], (2, 1): [0, 1, 2]}),
      4, [[(2, 0), (2, 0), (0, 2)], [(1, 0), (1, 0), (1, 1)], [(1, 0), (1, 0), (2, 0)]]),
+
+    # Box one point wide, 1 x 2 x 2, every verdict out.  Each line starts
+    # with min_xi = nx - 1 = 0 and max_xi = 0; the probe at x_left = 0 is
+    # out, the left walk steps to 1 and ends, the right walk steps to -1 and
+    # ends, and min <= max still holds: the line keeps point 0.  z0: y0
+    # (0, 0), first_x = 0, border_found; y1 (0, 0); last line: loosen from
+    # first_y = y_start = 0 -> y_start = 0.  z1 the same.  Four points.
+    ("one_point_wide_box", _mask(1, 2, 2, {}), 4, [[(0, 0), (0, 0)], [(0, 0), (0, 0)]]),
 
     # Empty box 3 x 2 x 2: every line visited, both walks run off: (2, 0);
     # no point found, so the cell falls back to CIC of its site
@@ -222,6 +231,9 @@ MUTANTS = {
     "no_early_z_exit": [(
         "            if (min_xi > max_xi && border_found)\n                z_step_done = 1;",
         "            if (min_xi > max_xi && border_found)\n                z_step_done = 0;")],
+    "run_starts_empty": [(
+        "            min_xi = cgp[0] - 1;\n            max_xi = 0;",
+        "            min_xi = cgp[0];\n            max_xi = -1;")],
     "loosen_probe_at_x_left": [(
         "if (!inside(ctx, first_x, yj, zi + 1))",
         "if (!inside(ctx, x_left, yj, zi + 1))")],

@@ -17,8 +17,12 @@
  *                                src/dense.cpp:30-202, include/tess/dense.hpp:75-98
  *   tess2_grid_step_params    <- GridStepParams()  src/dense.cpp:1712-1767
  *   tess2_block_grid_params   <- BlockGridParams() src/dense.cpp:575-648
- *   tess2_write_grid          <- WriteGrid() raw float32 output,
- *                                src/dense.cpp:751-870
+ *   tess2_write_grid          <- WriteGrid() raw float32 output of one global
+ *                                array, src/dense.cpp:751-870
+ *   tess2_write_blocks        <- WriteGrid() over blocks (subarray writes,
+ *                                ProjectGrid first when projecting),
+ *                                src/dense.cpp:751-1023
+ *   tess2_project_grid        <- ProjectGrid(), src/dense.cpp:881-1023
  * The staged (device-resident) calls have no reference counterpart: they
  * split dense() into upload / run / download so that a caller can keep the
  * tessellation resident in HBM across runs.
@@ -165,6 +169,27 @@ int tess2_dense_cells(tess2_ctx *ctx, int b, int *status, int *npts);
  * 2-D [y][x] plane. */
 int tess2_write_grid(const char *path, const float *density, const int *glo_num_idx,
                      int project);
+
+/* ProjectGrid() for blocks all held by the caller (one process): in
+ * ascending gid order, every block whose grid starts above z index 0 adds
+ * its 2-D plane, float += float point by point, into the plane of the block
+ * with the same x-y grid start at z index 0 (the root), as the reference
+ * does for blocks on one rank (src/dense.cpp:881-1023).  density[i] is block
+ * i's [y][x] plane from a projecting tess2_dense; block_min_idx /
+ * block_num_idx hold 3 ints per block (tess2_block_grid_params).  The roots'
+ * planes are updated in place. */
+int tess2_project_grid(int nblocks, const int *gids, float **density,
+                       const int *block_min_idx, const int *block_num_idx);
+
+/* WriteGrid() for blocks all held by the caller: the file is truncated,
+ * then every block's grid is written at its place in the global C-order
+ * array (x fastest).  With project, tess2_project_grid runs first (updating
+ * the roots' planes in place, as the reference's WriteGrid does) and only
+ * the blocks at z index 0 write their [y][x] planes into the global
+ * glo_num_idx[1] x glo_num_idx[0] plane (src/dense.cpp:800-846). */
+int tess2_write_blocks(const char *path, int nblocks, const int *gids, float **density,
+                       const int *block_min_idx, const int *block_num_idx,
+                       const int *glo_num_idx, int project);
 
 /* ---- staged (device-resident) interface ---- */
 

@@ -450,19 +450,22 @@ def exact_dense_multi(blocks, keys, gnum, mass=1.0):
     return dict(owner=owners, status=status, npts=counts, density=dens, cic=cic)
 
 
-def dense_multi(blocks, gnum, mass=1.0, eps=1e-4, cic=False):
+def dense_multi(blocks, gnum, mass=1.0, eps=1e-4, cic=False, project=False):
     """dense() over several blocks (src/dense.cpp:30-103): each block deposits
     its own cells' points that fall in its bounds and grid (IterateCells, in
     cell order); the others are sent to the block whose grid (BlockGridParams
     range) holds their global index and added there after its own deposits,
     in ascending source gid, then source cell, order (recvd_pts,
-    src/dense.cpp:166-202).  Points no block holds are dropped.
+    src/dense.cpp:166-202).  Points no block holds are dropped.  With
+    project, every block's density is its [y][x] plane (index(),
+    src/dense.cpp:1064-1089): its own points need only x and y in its grid
+    (their z is inside its bounds), routed points go by their 3-D index.
     blocks: list of Block with a .gid attribute.  Returns a dict."""
     order = sorted(range(len(blocks)), key=lambda i: blocks[i].gid)
     dmin = np.min([np.array(b.cs.bounds_min[:], np.float32) for b in blocks], axis=0)
     dmax = np.max([np.array(b.cs.bounds_max[:], np.float32) for b in blocks], axis=0)
     gmin, gmax, step = grid_step_params(dmin, dmax, gnum)
-    args = make_args(mass, dmin, dmax, gmin, step, eps, gnum)
+    args = make_args(mass, dmin, dmax, gmin, step, eps, gnum, project=project)
     grids = [block_grid_params(np.array(b.cs.bounds_min[:], np.float32),
                                np.array(b.cs.bounds_max[:], np.float32), gmin, step, eps,
                                dmin, dmax, gnum) for b in blocks]
@@ -495,3 +498,42 @@ def dense_multi(blocks, gnum, mass=1.0, eps=1e-4, cic=False):
             recvd_pts(idx, mm, mn, nm, args, dens[j])
     return dict(density=dens, status=stat, npts=npts, grids=grids, dropped=dropped,
                 grid_phys_mins=gmin, grid_step_size=step, args=args)
+
+
+def project_grid(result, gids):
+    """ProjectGrid() (src/dense.cpp:881-1023) on a dense_multi(project=True)
+    result, all blocks in one process: in ascending gid order, each block
+    whose grid starts above z index 0 adds its plane into the plane of the
+    z = 0 block of the same x-y grid start, float += float point by point.
+    Returns the planes after the sum (copies)."""
+    dens = [d.copy() for d in result["density"]]
+    grids = result["grids"]
+    for i in sorted(range(len(gids)), key=lambda k: gids[k]):
+        mn = grids[i][0]
+        if mn[2] == 0:
+            continue
+        root = [j for j, g in enumerate(grids) if g[0][0] == mn[0] and g[0][1] == mn[1]
+                and g[0][2] == 0][-1]
+        dens[root] = (dens[root] + dens[i]).astype(np.float32)
+    return dens
+
+
+def global_grid(result, gnum, project=False, gids=None):
+    """The raw file WriteGrid() writes (src/dense.cpp:751-870), as an array:
+    every block's grid at its place in the global C-order array; with
+    project, the z = 0 blocks' planes after project_grid."""
+    gx, gy, gz = (int(v) for v in _i3(gnum))
+    grids = result["grids"]
+    if project:
+        planes = project_grid(result, gids)
+        out = np.zeros((gy, gx), np.float32)
+        for d, (mn, _, nm) in zip(planes, grids):
+            if mn[2] != 0:
+                continue
+            out[mn[1]:mn[1] + nm[1], mn[0]:mn[0] + nm[0]] = d.reshape(nm[1], nm[0])
+        return out
+    out = np.zeros((gz, gy, gx), np.float32)
+    for d, (mn, _, nm) in zip(result["density"], grids):
+        out[mn[2]:mn[2] + nm[2], mn[1]:mn[1] + nm[1], mn[0]:mn[0] + nm[0]] = \
+            d.reshape(nm[2], nm[1], nm[0])
+    return out

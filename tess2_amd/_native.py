@@ -72,6 +72,10 @@ SIGNATURES = {
                               C.POINTER(F32P), C.POINTER(Stats)]),
     "tess2_dense_cells": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]),
     "tess2_write_grid": (C.c_int, [C.c_char_p, C.c_void_p, I32P, C.c_int]),
+    "tess2_project_grid": (C.c_int, [C.c_int, C.c_void_p, C.POINTER(F32P), C.c_void_p,
+                                     C.c_void_p]),
+    "tess2_write_blocks": (C.c_int, [C.c_char_p, C.c_int, C.c_void_p, C.POINTER(F32P),
+                                     C.c_void_p, C.c_void_p, I32P, C.c_int]),
     "tess2_stage": (C.c_int, [C.c_void_p, C.POINTER(Block), C.c_int]),
     "tess2_run": (C.c_int, [C.c_void_p, C.c_int, C.c_int, F32P, F32P, C.c_int, F32P,
                             C.c_float, C.c_float, I32P, C.c_int, C.c_int, C.c_int]),

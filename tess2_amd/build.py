@@ -14,6 +14,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libtess2_amd.so")
 SOURCES = ["api.hip", "common.hip", "cells_ref.hip", "exact.hip", "fold.hip"]
+HOST_SOURCES = ["grid_io.cpp"]      # host-only C++ (file output)
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
@@ -41,6 +42,15 @@ def build(verbose=False):
         src = os.path.join(CSRC, s)
         obj = os.path.join(objdir, s.replace(".hip", ".o"))
         cmd = [cc] + HIPCC_FLAGS + ["-c", src, "-o", obj]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        subprocess.check_call(cmd)
+        objs.append(obj)
+    for s in HOST_SOURCES:
+        src = os.path.join(CSRC, s)
+        obj = os.path.join(objdir, s.replace(".cpp", ".o"))
+        cmd = ["g++", "-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall", "-c", src, "-o",
+               obj]
         if verbose:
             print(" ".join(cmd), flush=True)
         subprocess.check_call(cmd)

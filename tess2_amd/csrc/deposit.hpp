@@ -48,12 +48,20 @@ __device__ __forceinline__ void push_point(long long lidx, float m, unsigned key
     }
 }
 
-__device__ __forceinline__ long long block_lidx(int gx, int gy, int gz, const GridP &gp)
+// local index of a global grid index in a block's grid, -1 outside it.  The
+// 2-D projection drops z from the index (index(), src/dense.cpp:1076-1089),
+// so a point of the block's own cells only needs its x and y in range (its z
+// is within the block's bounds already); a point routed to another block must
+// lie in that block's 3-D grid range (holds3d), as the reference sends it to
+// the neighbour whose bounds contain it in 3-D.
+__device__ __forceinline__ long long block_lidx(int gx, int gy, int gz, const GridP &gp,
+                                                bool holds3d = false)
 {
     const int lx = gx - gp.bmin_idx[0], ly = gy - gp.bmin_idx[1], lz = gz - gp.bmin_idx[2];
     if (lx < 0 || ly < 0 || lx >= gp.bnum_idx[0] || ly >= gp.bnum_idx[1]) return -1;
-    if (gp.project) return (long long)ly * gp.bnum_idx[0] + lx;
+    if (gp.project && !holds3d) return (long long)ly * gp.bnum_idx[0] + lx;
     if (lz < 0 || lz >= gp.bnum_idx[2]) return -1;
+    if (gp.project) return (long long)ly * gp.bnum_idx[0] + lx;
     return ((long long)lz * gp.bnum_idx[1] + ly) * gp.bnum_idx[0] + lx;
 }
 
@@ -79,7 +87,7 @@ __device__ __forceinline__ void deposit_point(int gx, int gy, int gz, float m, u
     if (rt) {
         for (int j = 0; j < rt->nblocks; ++j) {
             if (j == self) continue;
-            const long long lj = block_lidx(gx, gy, gz, rt->gps[j]);
+            const long long lj = block_lidx(gx, gy, gz, rt->gps[j], true);
             if (lj >= 0) {
                 const unsigned key = 0x80000000u | ((unsigned)self << 24) | ckey;
                 push_point(lj, m, key, rt->gps[j].div, rt->outs[j]);

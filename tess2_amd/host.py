@@ -9,6 +9,8 @@ library (tess2_amd/libtess2_amd.so):
   block_grid_params(...)                    src/dense.cpp:575-648
   dense(alg, ..., blocks)                   src/dense.cpp:30-103
   write_grid(path, density, glo_num_idx)    src/dense.cpp:751-870 (raw output)
+  project_grid(gids, density, ...)          src/dense.cpp:881-1023
+  write_blocks(path, gids, density, ...)    src/dense.cpp:751-1023 (block subarrays)
 
 Errors raise Tess2Error (the reference asserts or aborts instead).
 """
@@ -181,6 +183,45 @@ def write_grid(path, density, glo_num_idx, project=False):
                                  int(bool(project)))
     if r != 0:
         raise Tess2Error(f"write_grid: {N.ERRORS.get(r, r)}")
+
+
+def _block_arrays(gids, density, block_min_idx, block_num_idx):
+    n = len(gids)
+    if not (len(density) == len(block_min_idx) == len(block_num_idx) == n):
+        raise ValueError("one density, block_min_idx and block_num_idx per gid")
+    for d in density:
+        if not (isinstance(d, np.ndarray) and d.dtype == np.float32 and d.flags.c_contiguous):
+            raise TypeError("block densities must be C-contiguous float32 arrays "
+                            "(they are updated in place)")
+    g = np.ascontiguousarray(gids, np.int32)
+    mn = np.ascontiguousarray(np.asarray(block_min_idx).reshape(n, 3), np.int32)
+    nm = np.ascontiguousarray(np.asarray(block_num_idx).reshape(n, 3), np.int32)
+    ptrs = (N.F32P * n)()
+    for i, d in enumerate(density):
+        ptrs[i] = N.fptr(d)
+    return n, g, mn, nm, ptrs
+
+
+def project_grid(gids, density, block_min_idx, block_num_idx):
+    """ProjectGrid(): adds the [y][x] planes of blocks above z index 0 into the
+    plane of the z = 0 block of their x-y column, in ascending gid order; the
+    root planes (arrays in `density`) are updated in place."""
+    n, g, mn, nm, ptrs = _block_arrays(gids, density, block_min_idx, block_num_idx)
+    r = N.lib().tess2_project_grid(n, g.ctypes.data, ptrs, mn.ctypes.data, nm.ctypes.data)
+    if r != 0:
+        raise Tess2Error(f"project_grid: {N.ERRORS.get(r, r)}")
+
+
+def write_blocks(path, gids, density, block_min_idx, block_num_idx, glo_num_idx,
+                 project=False):
+    """WriteGrid() over blocks: each block's grid at its place in the global
+    raw float32 array (file truncated first); with project, ProjectGrid first
+    (root planes updated in place) and the z = 0 blocks' planes written."""
+    n, g, mn, nm, ptrs = _block_arrays(gids, density, block_min_idx, block_num_idx)
+    r = N.lib().tess2_write_blocks(str(path).encode(), n, g.ctypes.data, ptrs, mn.ctypes.data,
+                                   nm.ctypes.data, N.i3(glo_num_idx), int(bool(project)))
+    if r != 0:
+        raise Tess2Error(f"write_blocks: {N.ERRORS.get(r, r)}")
 
 
 class StagedRun:

@@ -242,8 +242,10 @@ def fp64_errors(hb, grid, gpu_density, exact_outs):
     ob = O.Block.from_host(hb)
     t0 = time.perf_counter()
     res = {"reference_scan": O.rel_error(gpu_density, O.dense_single_fp64(ob, grid))}
+    log(f"[cpu] FP64 scan density done ({time.perf_counter() - t0:.1f}s)")
     if exact_outs is not None:
         want = O.exact_dense(ob, grid)
+        log(f"[cpu] FP64 exact oracle done ({time.perf_counter() - t0:.1f}s)")
         for name, (own, den) in exact_outs.items():
             e = O.rel_error(den, want["density"])
             e["owner_identical"] = bool(np.array_equal(own, want["owner"]))

@@ -133,6 +133,23 @@ int tess2_fill_circumcenters(tess2_ctx *ctx, const tess2_tet_t *tets, int num_te
  * infinite cell (and for a particle in no tet). */
 int tess2_volumes(tess2_ctx *ctx, const tess2_block_t *block, float *out);
 
+/* volume() for every original cell, like tess2_volumes, allocating nothing.
+ * No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  Every array is in DEVICE memory, owned by the caller:
+ * particles (3*num_particles floats, xyz), tets (num_tets tet_t records,
+ * 16-byte aligned), vert_to_tet (num_particles), out (num_orig_particles).
+ * The library's working memory is one fixed array of 64 MiB per GPU, held
+ * from load to process end; a block needing more (about 16 bytes per
+ * particle, 16 per tet and 12 per original cell) returns TESS2_E_CAPACITY
+ * before any kernel runs, and out-of-range indices return TESS2_E_ARG.
+ * Runs on the context's stream and returns when the volumes are written.
+ * Every call on one GPU shares that array: calls that overlap on one GPU
+ * (two threads, two contexts) race and must be serialised by the caller. */
+int tess2_volumes_noalloc(tess2_ctx *ctx, const float *particles, int num_particles,
+                          int num_orig_particles, const tess2_tet_t *tets, int num_tets,
+                          const int *vert_to_tet, float *out);
+
 /* GridStepParams() / BlockGridParams() host helpers (same arguments). */
 void tess2_grid_step_params(int num_given_bounds, const float *given_mins,
                             const float *given_maxs, const float *data_mins,

@@ -13,10 +13,43 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
 using namespace t2;
+
+// ---------------------------------------------------------------------------
+// Device memory of the allocation-free entry point (tess2_volumes_noalloc):
+// one file-scope array, which every GPU the library runs on holds from the
+// code object's load until the process ends.  Host code reaches it only
+// through the address hipGetSymbolAddress returns for the current device
+// (scratch_ptr); the array's name on the host is not a device address.
+// ---------------------------------------------------------------------------
+static constexpr size_t NOALLOC_SCRATCH_BYTES = 64ull << 20;
+static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+
+static std::mutex g_scratch_mu;
+static std::vector<unsigned char *> g_scratch_ptr;   // per device; nullptr until looked up
+
+// once per device, with that device current (tess2_create)
+static hipError_t scratch_lookup(int device)
+{
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if ((int)g_scratch_ptr.size() <= device) g_scratch_ptr.resize((size_t)device + 1, nullptr);
+    if (g_scratch_ptr[(size_t)device]) return hipSuccess;
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e == hipSuccess) g_scratch_ptr[(size_t)device] = static_cast<unsigned char *>(p);
+    return e;
+}
+
+static unsigned char *scratch_ptr(int device)
+{
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    return device >= 0 && device < (int)g_scratch_ptr.size() ? g_scratch_ptr[(size_t)device]
+                                                             : nullptr;
+}
 
 namespace {
 
@@ -278,6 +311,11 @@ extern "C" int tess2_create(tess2_ctx **out, int device)
     if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess &&
         ncu > 0)
         c->ncu = ncu;
+    // the allocation-free entry point's scratch address on this device; a
+    // failed lookup only disables that entry point (it reports the error)
+    const hipError_t se = scratch_lookup(device);
+    if (se != hipSuccess)
+        fail(c, TESS2_E_HIP, "scratch lookup on device %d: %s", device, hipGetErrorString(se));
     *out = c;
     return 0;
 }
@@ -1287,6 +1325,77 @@ extern "C" int tess2_volumes(tess2_ctx *c, const tess2_block_t *block, float *ou
         return fail(c, TESS2_E_CAPACITY, "a cell exceeds the per-cell capacities");
     d.vol_valid = true;
     return tess2_fetch(c, 0, nullptr, out, nullptr);
+}
+
+extern "C" int tess2_volumes_noalloc(tess2_ctx *c, const float *particles, int num_particles,
+                                     int num_orig_particles, const tess2_tet_t *tets,
+                                     int num_tets, const int *vert_to_tet, float *out)
+{
+    if (!c) return TESS2_E_ARG;
+    const int np = num_particles, norig = num_orig_particles, nt = num_tets;
+    if (norig < 0 || np < norig || nt < 0) return fail(c, TESS2_E_ARG, "bad counts");
+    if (norig == 0) return 0;
+    if (!particles || !tets || !vert_to_tet || !out)
+        return fail(c, TESS2_E_ARG, "null array");
+    if (reinterpret_cast<uintptr_t>(tets) % 16 != 0)
+        return fail(c, TESS2_E_ARG, "tets must be 16-byte aligned");
+    HIPCHK(c, hipSetDevice(c->device));
+    unsigned char *base = scratch_ptr(c->device);
+    if (!base) return fail(c, TESS2_E_HIP, "no scratch address for device %d", c->device);
+    // scratch layout: counters, float4 particles, float4 circumcenters,
+    // status, point counts, deferred-cell list (one slot per cell, so it
+    // cannot overflow); 256-byte aligned regions
+    auto up = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    const size_t o_ctrs = 0, o_pts = up(o_ctrs + sizeof(unsigned) * C_NCTR),
+                 o_cc = up(o_pts + sizeof(float4) * (size_t)np),
+                 o_st = up(o_cc + sizeof(float4) * (size_t)nt),
+                 o_np = up(o_st + sizeof(int) * (size_t)norig),
+                 o_sl = up(o_np + sizeof(int) * (size_t)norig),
+                 need = up(o_sl + sizeof(int) * (size_t)norig);
+    if (need > NOALLOC_SCRATCH_BYTES)
+        return fail(c, TESS2_E_CAPACITY,
+                    "%d particles and %d tets need %zu bytes of scratch; the maximum is %zu",
+                    np, nt, need, NOALLOC_SCRATCH_BYTES);
+    unsigned *ctrs = reinterpret_cast<unsigned *>(base + o_ctrs);
+    float4 *pts = reinterpret_cast<float4 *>(base + o_pts);
+    float4 *cc = reinterpret_cast<float4 *>(base + o_cc);
+    const int4 *t4 = reinterpret_cast<const int4 *>(tets);
+    hipStream_t s = c->stream;
+    // the counters start at zero on every call (the array keeps the last
+    // call's contents); a kernel, not a memset
+    launch_fill_u32(ctrs, C_NCTR, 0u, s);
+    launch_check_block(t4, nt, vert_to_tet, np, &ctrs[C_ERR], 16u, s);
+    unsigned err = 0;
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipMemcpyAsync(&err, &ctrs[C_ERR], sizeof err, hipMemcpyDeviceToHost, s));
+    HIPCHK(c, hipStreamSynchronize(s));
+    if (err & 16u) return fail(c, TESS2_E_ARG, "a tet or vert_to_tet index is out of range");
+    launch_pack_points(particles, np, pts, s);
+    launch_circumcenters(pts, t4, nt, cc, s);
+    BlockIn in{};
+    in.pts = pts;
+    in.tets = t4;
+    in.v2t = vert_to_tet;
+    in.cc = cc;
+    in.order = nullptr;
+    in.ncells = norig;
+    in.ntets = nt;
+    CellOut o{};
+    o.vol = out;
+    o.status = reinterpret_cast<int *>(base + o_st);
+    o.npts = reinterpret_cast<int *>(base + o_np);
+    o.ctrs = ctrs;
+    o.slow = reinterpret_cast<int *>(base + o_sl);
+    o.slow_cap = (unsigned)norig;
+    o.ncells = norig;
+    GridP gp{};
+    launch_cells_ref(in, gp, o, 0, s);
+    launch_cells_ref_slow(in, gp, o, 0, s);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipMemcpyAsync(&err, &ctrs[C_ERR], sizeof err, hipMemcpyDeviceToHost, s));
+    HIPCHK(c, hipStreamSynchronize(s));
+    if (err & 4u) return fail(c, TESS2_E_CAPACITY, "a cell exceeds the per-cell capacities");
+    return 0;
 }
 
 extern "C" int tess2_dense(tess2_ctx *c, int alg, int num_given_bounds, const float *given_mins,

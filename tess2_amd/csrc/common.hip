@@ -101,6 +101,48 @@ void launch_fill_u32(unsigned *p, size_t n, unsigned v, hipStream_t s)
                        p + n4 * 4, ntail, v);
 }
 
+// particles given as xyz triples -> the padded float4 layout of the kernels
+__global__ void k_pack_points(const float *__restrict__ xyz, int np, float4 *__restrict__ pts)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= np) return;
+    pts[p] = make_float4(xyz[3 * p], xyz[3 * p + 1], xyz[3 * p + 2], 0.0f);
+}
+
+void launch_pack_points(const float *xyz, int np, float4 *pts, hipStream_t s)
+{
+    if (np <= 0) return;
+    hipLaunchKernelGGL(k_pack_points, dim3((np + 255) / 256), dim3(256), 0, s, xyz, np, pts);
+}
+
+// index ranges of a block held in device memory: every tet vertex in
+// [0, np), every neighbour in [-1, nt), every vert_to_tet entry in [-1, nt);
+// a violation sets bit `bit` of *err
+__global__ void k_check_block(const int4 *__restrict__ tets, int nt, const int *__restrict__ v2t,
+                              int np, unsigned *__restrict__ err, unsigned bit)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    bool bad = false;
+    if (i < nt) {
+        const int4 v = tets[2 * i], n = tets[2 * i + 1];
+        bad = (unsigned)v.x >= (unsigned)np || (unsigned)v.y >= (unsigned)np ||
+              (unsigned)v.z >= (unsigned)np || (unsigned)v.w >= (unsigned)np ||
+              n.x < -1 || n.x >= nt || n.y < -1 || n.y >= nt || n.z < -1 || n.z >= nt ||
+              n.w < -1 || n.w >= nt;
+    }
+    if (i < np) bad = bad || v2t[i] < -1 || v2t[i] >= nt;
+    if (bad) atomicOr(err, bit);
+}
+
+void launch_check_block(const int4 *tets, int nt, const int *v2t, int np, unsigned *err,
+                        unsigned bit, hipStream_t s)
+{
+    const int n = nt > np ? nt : np;
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_check_block, dim3((n + 255) / 256), dim3(256), 0, s, tets, nt, v2t, np,
+                       err, bit);
+}
+
 // per block (blockIdx.y): status histogram (5 codes) and total points of
 // deposited cells
 __global__ __launch_bounds__(256) void k_cell_stats(const CellOut *__restrict__ outs)

@@ -172,6 +172,11 @@ void launch_exact_tail(const BlockIn &in, const GridP &gp, const int *owner,
                        const int *const *stex_tab, const Router *rt, int self, int parts,
                        hipStream_t s);
 void launch_fill_u32(unsigned *p, size_t n, unsigned v, hipStream_t s);
+// xyz triples -> float4 (xyz, 0)
+void launch_pack_points(const float *xyz, int np, float4 *pts, hipStream_t s);
+// index-range check of a block in device memory: sets `bit` in *err on a violation
+void launch_check_block(const int4 *tets, int nt, const int *v2t, int np, unsigned *err,
+                        unsigned bit, hipStream_t s);
 // zero every block's density and count grids; circumcenters of every block
 void launch_block_prep(const BlockIn *ins, const CellOut *outs, int nb, int max_tets,
                        unsigned long long max_grid, bool circumcenters, hipStream_t s);

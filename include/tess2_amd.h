@@ -251,10 +251,16 @@ enum {
     TESS2_OPT_WAVES_PER_CU = 1,   /* persistent waves per CU of the multi-block cell kernel */
     TESS2_OPT_CHUNK_SMALL = 2,    /* cells per work item, blocks with < 20000 cells */
     TESS2_OPT_CHUNK_MEDIUM = 3,   /* cells per work item, blocks with < 100000 cells */
-    TESS2_OPT_CHUNK_LARGE = 4     /* cells per work item, larger blocks */
+    TESS2_OPT_CHUNK_LARGE = 4,    /* cells per work item, larger blocks */
+    TESS2_OPT_CELL_KERNEL = 5     /* single-block cell kernel: -1 (default) chosen per run
+                                   * (1, or 2 when cell boxes are large: 16 or more grid
+                                   * points per cell); 0 one cell per wave; 1 two cells per
+                                   * wave in the star / face / ring phase; 2 the same with
+                                   * the verdict masks of boxes up to 512 points */
 };
 
-/* Set / read one option.  Chunk sizes 1..4096, waves per CU 1..64. */
+/* Set / read one option.  Chunk sizes 1..4096, waves per CU 1..64, cell
+ * kernel -1..2. */
 int tess2_set_option(tess2_ctx *ctx, int option, int value);
 int tess2_get_option(tess2_ctx *ctx, int option, int *value);
 

@@ -91,6 +91,7 @@ SIGNATURES = {
 }
 
 OPT_GRAPH, OPT_WAVES_PER_CU, OPT_CHUNK_SMALL, OPT_CHUNK_MEDIUM, OPT_CHUNK_LARGE = 0, 1, 2, 3, 4
+OPT_CELL_KERNEL = 5
 
 
 def lib():

@@ -137,6 +137,10 @@ struct tess2_ctx {
     // batched schedule (tess2_autotune): chunk size per block size class and
     // persistent waves per CU
     int chunk_small = 16, chunk_medium = 32, chunk_large = 64, waves_per_cu = 28;
+    // single-block cell kernel: -1 = chosen per run (cell_kernel_for), 0 =
+    // one cell per wave, 1 / 2 = two cells per wave in the geometry phase
+    // (launch_cells_mc), 2 with the multi-word verdict masks
+    int cell_kernel = -1;
     // graph replay of the steady-state sequence (tess2_set_option)
     bool graph_on = false;
     hipGraphExec_t gexec = nullptr;
@@ -160,6 +164,16 @@ struct tess2_ctx {
 static bool same_args(const RunArgs &x, const RunArgs &y)
 {
     return memcmp(&x, &y, sizeof(RunArgs)) == 0;
+}
+
+// The single-block cell kernel of a run: the option when set, else two cells
+// per wave, with the multi-word verdict masks when cell boxes are large
+// (16 or more grid points per cell: 2.1M cells on 512^3, 20.2 vs 24.7 ms;
+// at 8 points per cell the masks cost 2 %).
+static int cell_kernel_for(const tess2_ctx *c, int ncells, unsigned long long ngrid)
+{
+    if (c->cell_kernel >= 0) return c->cell_kernel;
+    return ngrid >= 16ull * (unsigned long long)std::max(ncells, 1) ? 2 : 1;
 }
 
 static void drop_graph(tess2_ctx *c)
@@ -881,7 +895,11 @@ static int enqueue_kernels(tess2_ctx *c, const RunArgs &a)
             if (a.want_vol) {
                 // volume() as the reference computes it
                 CellOut ov = make_out(d, true);
-                launch_cells_ref(in, gp, ov, 0, s);
+                const int ck = nb > 1 ? 0 : cell_kernel_for(c, d.norig, d.ngrid);
+                if (ck == 0)
+                    launch_cells_ref(in, gp, ov, 0, s);
+                else if (launch_cells_mc(in, gp, ov, 0, ck, s))
+                    return fail(c, TESS2_E_ARG, "unknown cell kernel %d", ck);
                 launch_cells_ref_slow(in, gp, ov, 0, s);
             }
             launch_hash_build(d.pts, d.np, d.hash, d.hstart, d.hfill, d.hsums, d.hbk, d.hsorted,
@@ -915,7 +933,11 @@ static int enqueue_kernels(tess2_ctx *c, const RunArgs &a)
         if (ev) HIPCHK(c, hipEventRecord(c->ev_k1, s));
     } else if (nb == 1) {
         if (ev) HIPCHK(c, hipEventRecord(c->ev_k0, s));
-        launch_cells_ref(c->h_ins[0], c->h_gps[0], c->h_outs[0], 1, s);
+        const int ck = cell_kernel_for(c, c->blocks[0].norig, c->blocks[0].ngrid);
+        if (ck == 0)
+            launch_cells_ref(c->h_ins[0], c->h_gps[0], c->h_outs[0], 1, s);
+        else if (launch_cells_mc(c->h_ins[0], c->h_gps[0], c->h_outs[0], 1, ck, s))
+            return fail(c, TESS2_E_ARG, "unknown cell kernel %d", ck);
         if (ev) HIPCHK(c, hipEventRecord(c->ev_k1, s));
         launch_cells_ref_slow(c->h_ins[0], c->h_gps[0], c->h_outs[0], 1, s);
     } else {
@@ -1117,6 +1139,11 @@ extern "C" int tess2_set_option(tess2_ctx *c, int option, int value)
         if (value != c->waves_per_cu) c->gen++;
         c->waves_per_cu = value;
         return 0;
+    case TESS2_OPT_CELL_KERNEL:
+        if (value < -1 || value > 2) return fail(c, TESS2_E_ARG, "cell kernel: -1..2");
+        if (value != c->cell_kernel) c->gen++;   // a recorded graph is recorded again
+        c->cell_kernel = value;
+        return 0;
     case TESS2_OPT_CHUNK_SMALL: chunk = &c->chunk_small; break;
     case TESS2_OPT_CHUNK_MEDIUM: chunk = &c->chunk_medium; break;
     case TESS2_OPT_CHUNK_LARGE: chunk = &c->chunk_large; break;
@@ -1137,6 +1164,7 @@ extern "C" int tess2_get_option(tess2_ctx *c, int option, int *value)
     case TESS2_OPT_CHUNK_SMALL: *value = c->chunk_small; return 0;
     case TESS2_OPT_CHUNK_MEDIUM: *value = c->chunk_medium; return 0;
     case TESS2_OPT_CHUNK_LARGE: *value = c->chunk_large; return 0;
+    case TESS2_OPT_CELL_KERNEL: *value = c->cell_kernel; return 0;
     default: return fail(c, TESS2_E_ARG, "unknown option %d", option);
     }
 }
@@ -1302,7 +1330,9 @@ extern "C" int tess2_volumes(tess2_ctx *c, const tess2_block_t *block, float *ou
     BlockIn in = make_in(d);
     CellOut o = make_out(d, true);
     GridP gp{};
-    launch_cells_ref(in, gp, o, 0, s);
+    const int ck = c->cell_kernel >= 0 ? c->cell_kernel : 1;
+    if (ck == 0) launch_cells_ref(in, gp, o, 0, s);
+    else launch_cells_mc(in, gp, o, 0, ck, s);
     launch_cells_ref_slow(in, gp, o, 0, s);
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipMemcpyAsync(d.hm->ctrs, d.ctrs, sizeof(unsigned) * C_NCTR, hipMemcpyDeviceToHost,

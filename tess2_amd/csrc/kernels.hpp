@@ -184,6 +184,10 @@ void launch_cells_ref(const BlockIn &in, const GridP &gp, const CellOut &o,
                       int want_density, hipStream_t s);
 void launch_cells_ref_slow(const BlockIn &in, const GridP &gp, const CellOut &o,
                            int want_density, hipStream_t s);
+// several cells per wave in the geometry phase (cells_mc.hip); variant 1..4
+// picks cells per wave and capacities; returns -1 for an unknown variant
+int launch_cells_mc(const BlockIn &in, const GridP &gp, const CellOut &o, int want_density,
+                    int variant, hipStream_t s);
 void launch_cic(const BlockIn &in, const GridP &gp, const CellOut &o, const Router *rt_dev,
                 int self, hipStream_t s);
 void launch_cells_ref_batched(const BlockIn *ins, const GridP *gps, const CellOut *outs,

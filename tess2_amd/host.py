@@ -301,7 +301,7 @@ class StagedRun:
 
     OPTIONS = {"graph": N.OPT_GRAPH, "waves_per_cu": N.OPT_WAVES_PER_CU,
                "chunk_small": N.OPT_CHUNK_SMALL, "chunk_medium": N.OPT_CHUNK_MEDIUM,
-               "chunk_large": N.OPT_CHUNK_LARGE}
+               "chunk_large": N.OPT_CHUNK_LARGE, "cell_kernel": N.OPT_CELL_KERNEL}
 
     def set_option(self, name, value):
         """Launch option (graph replay, persistent waves, chunk sizes)."""

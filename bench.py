@@ -384,7 +384,11 @@ def main():
     tf = os.path.join(ROOT, "profiles", f"traffic_{c['name']}.json")
     if os.path.exists(tf):
         traffic = json.load(open(tf)).get("hbm_bytes_per_launch")
-    kernel = "k_cells_ref_batched" if multi else "k_cells_ref"
+    if multi:
+        kernel = "k_cells_ref_batched"
+    else:
+        ck = run.get_option("cell_kernel")
+        kernel = "k_cells_ref" if ck == 0 else "k_cells_mc"
     cfg = {"workload": c["name"], "blocks": len(blocks), "particles": N, "ghosts": N_all - N,
            "tets": T, "grid": list(grid), "grid_points": G, "assignment": "reference_scan",
            "estimator": "DENSE_TESS", "steady_state": "hip_graph" if graph else "eager",

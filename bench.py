@@ -301,6 +301,10 @@ def main():
                     help="run the oracle over the whole block (full-grid density check "
                          "and FP64 error report) when it is estimated to take at most this "
                          "many seconds; 0 = always sample")
+    ap.add_argument("--ghost-check", action="store_true",
+                    help="multi-block workloads: also build the ghost layers by the exchange "
+                         "rounds with the GPU's incomplete_cells test and check that every "
+                         "original star closes")
     ap.add_argument("--no-graph", action="store_true",
                     help="time eager launches instead of graph replays (for tracing tools)")
     ap.add_argument("--save-block", default=None,
@@ -433,7 +437,41 @@ def main():
         if not multi and sampled[3] is not None:
             out["fp64_error"] = fp64_errors(blocks[0], grid, gpu_density, exact_outs)
             log(f"[cpu] errors against the FP64 oracle: {out['fp64_error']}")
+    if args.ghost_check and multi:
+        out["ghost_exchange"] = ghost_check(args.config)
     print(json.dumps(out), flush=True)
+
+
+def ghost_check(cfg):
+    """The workload's blocks with ghost layers from the exchange rounds
+    (incomplete_cells test on the GPU each round, tessellation on the host)
+    instead of a fixed ghost width; every original star is certified as its
+    Delaunay star among all particles (closed, empty circumspheres)."""
+    from tess2_amd import host, ingest, synthetic
+    c = synthetic.CONFIGS[cfg]
+    workers = ingest.default_workers()
+    pts, bid, counts, bounds = synthetic.plummer_blocks(c["nblocks_side"], c["nmin"], c["nmax"],
+                                                        seed=cfg)
+    t0 = time.time()
+    gpu_s = [0.0]
+
+    def sel(hb, boxes, first):
+        t = time.perf_counter()
+        m = host.ghost_sends(hb, boxes, first)
+        gpu_s[0] += time.perf_counter() - t
+        return m
+
+    log("[ghosts] exchange rounds with the GPU test")
+    blocks, rounds, sent = ingest.exchanged_blocks(pts, bid, bounds, sel, workers=workers,
+                                                   log=log)
+    t1 = time.time()
+    log(f"[ghosts] {rounds} rounds, sent {sent} ({t1 - t0:.1f}s); certifying the stars")
+    res = ingest.stars_certified(blocks, pts, bid, workers=workers, log=log)
+    res.update(rounds=rounds, sent_per_round=sent, blocks=len(blocks),
+               ghosts=int(sum(b.num_particles - b.num_orig for b in blocks)),
+               exchange_s=t1 - t0, gpu_test_s=gpu_s[0])
+    log(f"[ghosts] {res}")
+    return res
 
 
 if __name__ == "__main__":

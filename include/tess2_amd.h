@@ -23,6 +23,8 @@
  *                                ProjectGrid first when projecting),
  *                                src/dense.cpp:751-1023
  *   tess2_project_grid        <- ProjectGrid(), src/dense.cpp:881-1023
+ *   tess2_ghost_sends         <- the test of incomplete_cells(),
+ *                                src/tess.cpp:492-628
  * The staged (device-resident) calls have no reference counterpart: they
  * split dense() into upload / run / download so that a caller can keep the
  * tessellation resident in HBM across runs.
@@ -149,6 +151,21 @@ int tess2_volumes(tess2_ctx *ctx, const tess2_block_t *block, float *out);
 int tess2_volumes_noalloc(tess2_ctx *ctx, const float *particles, int num_particles,
                           int num_orig_particles, const tess2_tet_t *tets, int num_tets,
                           const int *vert_to_tet, float *out);
+
+/* The test of incomplete_cells() (src/tess.cpp:492-628) for one block of a
+ * tessellation round: for every original particle p, bit i of sends[p]
+ * (num_orig_particles entries) is set when p must be sent to neighbour i as
+ * a ghost: a tet with p has a circumsphere reaching neighbour box i (unless
+ * the sphere lies deep inside the block's own bounds_min/bounds_max), or p
+ * is on a convex-hull facet whose outside reaches box i (side_of_plane,
+ * src/tet.cpp:70-119).  neighbor_boxes: 6 floats per neighbour (min xyz,
+ * max xyz), already wrapped for periodic neighbours; at most 64; neighbours
+ * below first_neighbor (the reference's last_neighbor) are not tested.
+ * The rounds themselves (send, tessellate again, stop when nothing is sent,
+ * src/tess.cpp:79-98) are the caller's. */
+int tess2_ghost_sends(tess2_ctx *ctx, const tess2_block_t *block, int num_neighbors,
+                      const float *neighbor_boxes, int first_neighbor,
+                      unsigned long long *sends);
 
 /* GridStepParams() / BlockGridParams() host helpers (same arguments). */
 void tess2_grid_step_params(int num_given_bounds, const float *given_mins,

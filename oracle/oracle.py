@@ -84,6 +84,8 @@ def lib():
         L.o_iterate_cells_cic.argtypes = [C.POINTER(OBlock), C.POINTER(OArgs), I32P, I32P,
                                           F32P, vp, vp, vp, C.c_long, vp]
         L.o_recvd_pts.argtypes = [I32P, F64P, C.c_long, I32P, I32P, C.POINTER(OArgs), F32P]
+        L.o_ghost_sends.argtypes = [C.POINTER(OBlock), C.c_int, F32P, C.c_int,
+                                    np.ctypeslib.ndpointer(np.uint64, flags="C_CONTIGUOUS")]
         L.o_exact_owner.argtypes = [F32P, C.c_int, I32P, F32P, F32P, I32P]
         L.o_exact_owner_off.argtypes = [F32P, C.c_int, I32P, I32P, F32P, F32P, I32P]
         L.o_exact_dense.argtypes = [C.POINTER(OBlock), C.POINTER(OArgs), I32P, I32P, I32P,
@@ -301,6 +303,18 @@ def recvd_pts(idx3, mass, bmin_idx, bnum_idx, args, density):
     mass = np.ascontiguousarray(mass, dtype=np.float64)
     lib().o_recvd_pts(idx3, mass, mass.size, _i3(bmin_idx), _i3(bnum_idx),
                       C.byref(args), density)
+
+
+def ghost_sends(block, boxes, first=0):
+    """incomplete_cells() (src/tess.cpp:492-628): for every original particle
+    of the block a bit mask of the neighbour boxes (n x 6: min xyz, max xyz,
+    already wrapped) it must be sent to; neighbours before `first` are not
+    tested (the reference's last_neighbor)."""
+    bx = np.ascontiguousarray(boxes, dtype=np.float32).reshape(-1, 6)
+    assert bx.shape[0] <= 64
+    out = np.zeros(block.num_orig, np.uint64)
+    lib().o_ghost_sends(C.byref(block.cs), bx.shape[0], bx.reshape(-1), int(first), out)
+    return out
 
 
 def exact_owner(particles, gnum, step, gmin):

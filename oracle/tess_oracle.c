@@ -1231,3 +1231,138 @@ void o_exact_dense(const o_block_t *b, const o_args_t *a, const int *bmin_idx,
     free(own_part);
     free(own_cell);
 }
+
+/* ------------------------------------------------------------------------ */
+/* ghost selection: incomplete_cells(), src/tess.cpp:492-628                */
+/* ------------------------------------------------------------------------ */
+
+/* dot(u, p, x) = u . (p - x), accumulated from 0, src/tet.cpp:21-29 */
+static float o_dot3(const float *u, const float *p, const float *x)
+{
+    float res = 0.0f;
+    for (int i = 0; i < 3; ++i)
+        res += u[i] * (p[i] - x[i]);
+    return res;
+}
+
+/* side_of_plane(), src/tet.cpp:70-119: does a point of the box lie on the
+ * other side of the facet opposite vertex j than vertex j itself?  A flat
+ * facet (vertex j on its plane) counts as yes. */
+static int o_side_of_plane(const float *bmin, const float *bmax, const o_tet_t *tet,
+                           const float *particles, int j)
+{
+    int idx[4] = {0, 1, 2, 3};
+    int tmp = idx[j];
+    idx[j] = idx[3];
+    idx[3] = tmp;
+    const float *x = &particles[3 * tet->verts[idx[0]]];
+    const float *y = &particles[3 * tet->verts[idx[1]]];
+    const float *z = &particles[3 * tet->verts[idx[2]]];
+    const float *p = &particles[3 * tet->verts[idx[3]]];
+    float u[3], v[3], n[3];
+    for (int i = 0; i < 3; ++i) {
+        u[i] = y[i] - x[i];
+        v[i] = z[i] - x[i];
+    }
+    o_cross(n, u, v);
+    float res = o_dot3(n, p, x);
+    int sign = 0;
+    if (res > 0)
+        sign = 1;
+    else if (res < 0)
+        sign = -1;
+    if (sign == 0)
+        return 1;
+    res = 0.0f;
+    for (int i = 0; i < 3; ++i) {
+        if ((sign > 0) != (n[i] < 0))
+            res += n[i] * (bmin[i] - x[i]);
+        else
+            res += n[i] * (bmax[i] - x[i]);
+    }
+    return (sign > 0) != (res > 0);
+}
+
+/* Distance from a point to a box, 0 inside: per axis the larger of
+ * (min - p) and (p - max) when positive, squared and summed in float, then
+ * the square root.  This is DIY's distance(Bounds, Point), used at
+ * src/tess.cpp:568; DIY is not part of the reference tree (an unpinned
+ * GitHub checkout, .build.yml:11-17), so the float rounding of this helper
+ * is restated from its published source, not pinned by a reference build. */
+static float o_box_distance(const float *bmin, const float *bmax, const float *p)
+{
+    float res = 0.0f;
+    for (int i = 0; i < 3; ++i) {
+        float diff = 0.0f, d;
+        d = bmin[i] - p[i];
+        if (d > diff) diff = d;
+        d = p[i] - bmax[i];
+        if (d > diff) diff = d;
+        res += diff * diff;
+    }
+    return sqrtf(res);
+}
+
+void o_ghost_sends(const o_block_t *b, int nnb, const float *boxes, int first,
+                   unsigned long long *sends)
+{
+    const int norig = b->num_orig_particles;
+    for (int p = 0; p < norig; ++p)
+        sends[p] = 0ull;
+    for (int t = 0; t < b->num_tets; t++) {
+        const o_tet_t *tet = &b->tets[t];
+        int j;
+        for (j = 0; j < 4; ++j)
+            if (tet->verts[j] < norig)
+                break;
+        if (j == 4)     /* no local particle in the tet */
+            continue;
+        float center[3];
+        o_circumcenter(center, tet, b->particles);
+        const float rad = o_distance(center, &b->particles[3 * tet->verts[0]]);
+        /* convex hull facets with a local particle (:516-547) */
+        for (j = 0; j < 4; ++j) {
+            if (tet->tets[j] != -1)
+                continue;
+            int k;
+            for (k = 0; k < 4; ++k) {
+                if (k == j)
+                    continue;
+                if (tet->verts[k] < norig)
+                    break;
+            }
+            if (k == 4)
+                continue;
+            for (int i = first; i < nnb; ++i) {
+                if (!o_side_of_plane(&boxes[6 * i], &boxes[6 * i + 3], tet, b->particles, j))
+                    continue;
+                for (int q = 0; q < 4; q++) {
+                    if (q == j)
+                        continue;
+                    const int pp = tet->verts[q];
+                    if (pp < norig)
+                        sends[pp] |= 1ull << i;
+                }
+            }
+        }
+        /* a circumsphere too deep inside the block cannot stick out (:550-557) */
+        for (j = 0; j < 3; ++j) {
+            if (center[j] - b->bounds_min[j] <= rad)
+                break;
+            if (b->bounds_max[j] - center[j] <= rad)
+                break;
+        }
+        if (j == 3)
+            continue;
+        /* neighbour boxes within the circumradius (:560-578) */
+        for (int i = first; i < nnb; ++i) {
+            if (o_box_distance(&boxes[6 * i], &boxes[6 * i + 3], center) <= rad) {
+                for (int q = 0; q < 4; q++) {
+                    const int pp = tet->verts[q];
+                    if (pp < norig)
+                        sends[pp] |= 1ull << i;
+                }
+            }
+        }
+    }
+}

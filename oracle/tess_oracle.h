@@ -153,6 +153,14 @@ void o_recvd_pts(const int *idx3, const double *mass, long n,
                  const int *block_min_idx, const int *block_num_idx,
                  const o_args_t *a, float *density);
 
+/* ---- ghost selection (src/tess.cpp:492-628) ----
+ * incomplete_cells() for one block: for every original particle p, bit i of
+ * sends[p] is set when p must go to neighbour i (first <= i < nnb <= 64).
+ * boxes: 6 floats per neighbour (min xyz, max xyz), already wrapped; the
+ * block's own box is b->bounds_min / bounds_max. */
+void o_ghost_sends(const o_block_t *b, int nnb, const float *boxes, int first,
+                   unsigned long long *sends);
+
 /* ---- exact Voronoi assignment (FP64 oracle for the exact mode) ----
  * Owner of every grid point of a (nx,ny,nz) grid at pos = idx*step + gmin
  * (float32, as idx2phys): the nearest site among all particles, ties to the

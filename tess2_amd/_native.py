@@ -62,6 +62,8 @@ SIGNATURES = {
     "tess2_fill_circumcenters": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p,
                                            C.c_int, C.c_void_p]),
     "tess2_volumes": (C.c_int, [C.c_void_p, C.POINTER(Block), C.c_void_p]),
+    "tess2_ghost_sends": (C.c_int, [C.c_void_p, C.POINTER(Block), C.c_int, C.c_void_p, C.c_int,
+                                    C.c_void_p]),
     "tess2_volumes_noalloc": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_void_p,
                                         C.c_int, C.c_void_p, C.c_void_p]),
     "tess2_grid_step_params": (None, [C.c_int, F32P, F32P, F32P, F32P, F32P, F32P, F32P,

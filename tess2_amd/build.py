@@ -13,7 +13,8 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libtess2_amd.so")
-SOURCES = ["api.hip", "common.hip", "cells_ref.hip", "cells_mc.hip", "exact.hip", "fold.hip"]
+SOURCES = ["api.hip", "common.hip", "cells_ref.hip", "cells_mc.hip", "exact.hip", "fold.hip",
+           "ghosts.hip"]
 HOST_SOURCES = ["grid_io.cpp"]      # host-only C++ (file output)
 
 HIPCC_FLAGS = [

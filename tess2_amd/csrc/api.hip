@@ -1428,6 +1428,38 @@ extern "C" int tess2_volumes_noalloc(tess2_ctx *c, const float *particles, int n
     return 0;
 }
 
+extern "C" int tess2_ghost_sends(tess2_ctx *c, const tess2_block_t *block, int num_neighbors,
+                                 const float *neighbor_boxes, int first_neighbor,
+                                 unsigned long long *sends)
+{
+    if (!c || !block || num_neighbors < 0 || num_neighbors > 64 || first_neighbor < 0 ||
+        (num_neighbors > 0 && !neighbor_boxes) || (block->num_orig_particles > 0 && !sends))
+        return fail(c, TESS2_E_ARG, "bad arguments (at most 64 neighbours)");
+    int r = tess2_stage(c, block, 1);
+    if (r) return r;
+    DevBlock &d = c->blocks[0];
+    if (d.norig == 0) return 0;
+    float *dbox = nullptr;
+    unsigned long long *dsend = nullptr;
+    if ((r = dalloc(c, &dbox, (size_t)std::max(6 * num_neighbors, 1))) ||
+        (r = dalloc(c, &dsend, (size_t)d.norig)))
+        return r;
+    hipStream_t s = c->stream;
+    if (num_neighbors > 0)
+        HIPCHK(c, hipMemcpyAsync(dbox, neighbor_boxes, sizeof(float) * 6 * num_neighbors,
+                                 hipMemcpyHostToDevice, s));
+    launch_fill_u32(reinterpret_cast<unsigned *>(dsend), 2 * (size_t)d.norig, 0u, s);
+    launch_ghost_sends(d.pts, d.tets, d.nt, d.norig, d.bmin, d.bmax, dbox, num_neighbors,
+                       first_neighbor, dsend, s);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipMemcpyAsync(sends, dsend, sizeof(unsigned long long) * d.norig,
+                             hipMemcpyDeviceToHost, s));
+    HIPCHK(c, hipStreamSynchronize(s));
+    (void)hipFree(dbox);
+    (void)hipFree(dsend);
+    return 0;
+}
+
 extern "C" int tess2_dense(tess2_ctx *c, int alg, int num_given_bounds, const float *given_mins,
                            const float *given_maxs, int project, const float *proj_plane,
                            float mass, float *data_mins, float *data_maxs, float *grid_phys_mins,

@@ -172,6 +172,11 @@ void launch_exact_tail(const BlockIn &in, const GridP &gp, const int *owner,
                        const int *const *stex_tab, const Router *rt, int self, int parts,
                        hipStream_t s);
 void launch_fill_u32(unsigned *p, size_t n, unsigned v, hipStream_t s);
+// ghost selection (incomplete_cells): bit i of sends[p] for neighbour box i
+// (6 floats each, [first, nnb)), for original particles p < norig
+void launch_ghost_sends(const float4 *pts, const int4 *tets, int ntets, int norig,
+                        const float *bmin, const float *bmax, const float *boxes, int nnb,
+                        int first, unsigned long long *sends, hipStream_t s);
 // xyz triples -> float4 (xyz, 0)
 void launch_pack_points(const float *xyz, int np, float4 *pts, hipStream_t s);
 // index-range check of a block in device memory: sets `bit` in *err on a violation

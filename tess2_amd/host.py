@@ -11,6 +11,7 @@ library (tess2_amd/libtess2_amd.so):
   write_grid(path, density, glo_num_idx)    src/dense.cpp:751-870 (raw output)
   project_grid(gids, density, ...)          src/dense.cpp:881-1023
   write_blocks(path, gids, density, ...)    src/dense.cpp:751-1023 (block subarrays)
+  ghost_sends(block, boxes, first)          src/tess.cpp:492-628 (incomplete_cells test)
 
 Errors raise Tess2Error (the reference asserts or aborts instead).
 """
@@ -222,6 +223,20 @@ def write_blocks(path, gids, density, block_min_idx, block_num_idx, glo_num_idx,
                                    nm.ctypes.data, N.i3(glo_num_idx), int(bool(project)))
     if r != 0:
         raise Tess2Error(f"write_blocks: {N.ERRORS.get(r, r)}")
+
+
+def ghost_sends(block, boxes, first=0, ctx=None) -> np.ndarray:
+    """The test of incomplete_cells() (src/tess.cpp:492-628) on the GPU: for
+    every original particle of the block, a bit mask of the neighbour boxes
+    (n x 6: min xyz, max xyz, wrapped; n <= 64) it must be sent to;
+    neighbours before `first` are not tested."""
+    ctx = ctx or context()
+    b, keep = N.make_block(block)
+    bx = np.ascontiguousarray(boxes, dtype=np.float32).reshape(-1, 6)
+    out = np.zeros(block.num_orig, np.uint64)
+    ctx.check(N.lib().tess2_ghost_sends(ctx.h, C.byref(b), bx.shape[0], bx.ctypes.data,
+                                        int(first), out.ctypes.data), "ghost_sends")
+    return out
 
 
 class StagedRun:

@@ -366,3 +366,210 @@ def decomposed_blocks(points: np.ndarray, block_id: np.ndarray, bounds, ghost_wi
             log(f"  block {b}: {norig} + {allp.shape[0] - norig} ghosts, {tets.shape[0]} tets, "
                 f"{dt:.1f}s")
     return blocks
+
+
+def adjacent_blocks(bounds, b) -> list:
+    """Blocks whose closed boxes touch block b's (face, edge or corner
+    neighbours of a regular decomposition), ascending index."""
+    lo, hi = (np.asarray(x, np.float32) for x in bounds[b])
+    out = []
+    for n, (l2, h2) in enumerate(bounds):
+        if n != b and np.all(np.asarray(l2) <= hi) and np.all(np.asarray(h2) >= lo):
+            out.append(n)
+    return out
+
+
+def exchanged_blocks(points: np.ndarray, block_id: np.ndarray, bounds, select,
+                     workers: int = 1, log=None, max_rounds: int = 16):
+    """DIY-style blocks whose ghosts come from the reference's exchange
+    rounds, non-periodic (tess(), src/tess.cpp:45-116; delaunay() :351-457):
+    every block starts from its own particles; each round tessellates every
+    block (originals + the ghosts received so far) and asks `select` which
+    originals must go to which neighbour -- the test of incomplete_cells()
+    (:492-628) -- then sends those a neighbour does not hold yet.  A block's
+    links start as its adjacent blocks; after each round they grow by the
+    neighbours' own adjacent blocks, and the next round tests only the new
+    ones (the reference's last_neighbor).  Stops after a round that sends
+    nothing.  select(block, boxes, first) -> uint64 mask per original
+    particle (bit i: neighbour i of boxes, an n x 6 array of min / max).
+    Returns (blocks with ghost homes, rounds, particles sent per round)."""
+    points = np.ascontiguousarray(points, dtype=np.float32)
+    block_id = np.asarray(block_id)
+    nb = len(bounds)
+    own = [np.nonzero(block_id == b)[0] for b in range(nb)]
+    home_idx = np.zeros(points.shape[0], np.int32)
+    for b in range(nb):
+        home_idx[own[b]] = np.arange(own[b].size, dtype=np.int32)
+    adj = [adjacent_blocks(bounds, b) for b in range(nb)]
+    links = [list(a) for a in adj]
+    last = [0] * nb
+    ghosts = [[] for _ in range(nb)]
+    have = [set() for _ in range(nb)]
+    box = np.array([[*np.asarray(lo, np.float32), *np.asarray(hi, np.float32)]
+                    for lo, hi in bounds], np.float32)
+    sent_per_round = []
+    blocks = []
+    for rnd in range(1, max_rounds + 1):
+        jobs = []
+        for b in range(nb):
+            idx = np.concatenate([own[b], np.asarray(ghosts[b], np.int64)]).astype(np.int64)
+            lo, hi = (np.asarray(x, np.float32) for x in bounds[b])
+            jobs.append((points[idx], own[b].size, lo, hi, b))
+        if workers > 1 and nb > 1:
+            from concurrent.futures import ProcessPoolExecutor
+            with ProcessPoolExecutor(max_workers=workers) as ex:
+                results = list(ex.map(_block_job, jobs))
+        else:
+            results = [_block_job(j) for j in jobs]
+        blocks = []
+        for (allp, norig, lo, hi, b), (tets, dt) in zip(jobs, results):
+            g = np.asarray(ghosts[b], np.int64)
+            blocks.append(HostBlock(particles=allp, tets=tets,
+                                    vert_to_tet=vert_to_tet(tets, allp.shape[0]), num_orig=norig,
+                                    bounds_min=lo, bounds_max=hi, gid=b,
+                                    ghost_gid=block_id[g].astype(np.int32),
+                                    ghost_cell=home_idx[g], info=dict(qhull_s=dt, round=rnd)))
+        sent = 0
+        new = [[] for _ in range(nb)]
+        for b in range(nb):
+            if last[b] >= len(links[b]):
+                continue
+            masks = np.asarray(select(blocks[b], box[links[b]], last[b]), np.uint64)
+            for p in np.nonzero(masks)[0]:
+                m = int(masks[p])
+                gidx = int(own[b][p])
+                for i in range(last[b], len(links[b])):
+                    if (m >> i) & 1:
+                        d = links[b][i]
+                        if gidx not in have[d]:
+                            have[d].add(gidx)
+                            new[d].append(gidx)
+                            sent += 1
+        sent_per_round.append(sent)
+        if log:
+            log(f"  round {rnd}: {sent} particles sent")
+        if sent == 0:
+            return blocks, rnd, sent_per_round
+        for d in range(nb):
+            ghosts[d].extend(new[d])
+        # links grow by the neighbours' adjacent blocks; only those are tested next
+        for b in range(nb):
+            grown = sorted(set(n2 for n in links[b] for n2 in adj[n]) - set(links[b]) - {b})
+            last[b] = len(links[b])
+            links[b].extend(grown)
+            if len(links[b]) > 64:
+                raise ValueError("more than 64 neighbour links")
+    raise RuntimeError(f"ghost exchange did not settle in {max_rounds} rounds")
+
+
+def _sorted_keys(t: np.ndarray):
+    """Rows of sorted vertex ids as two int64 keys each, sorted, unique."""
+    if t.shape[0] == 0:
+        return np.zeros((0, 2), np.int64)
+    n = np.int64(t.max()) + 1
+    k = np.stack([t[:, 0] * n + t[:, 1], t[:, 2] * n + t[:, 3]], axis=1)
+    o = np.lexsort((k[:, 1], k[:, 0]))
+    k = k[o]
+    keep = np.ones(k.shape[0], bool)
+    keep[1:] = np.any(k[1:] != k[:-1], axis=1)
+    return k[keep]
+
+
+def stars_closed(blocks, points, block_id, workers: int = 1, tets=None) -> dict:
+    """Whether every original particle's star in its block equals its star in
+    the Delaunay tessellation of all points (tets, computed when not given):
+    the tets with at least one of a block's originals, as sorted global
+    vertex ids, are compared as sets."""
+    points = np.ascontiguousarray(points, dtype=np.float32)
+    block_id = np.asarray(block_id)
+    if tets is None:
+        tets = tessellate(points, workers=workers)
+    gt = np.sort(tets[:, :4].astype(np.int64), axis=1)
+    bad_blocks, checked = [], 0
+    for hb in blocks:
+        b = hb.gid
+        own = np.nonzero(block_id == b)[0]
+        gmap = np.concatenate([own, _ghost_globals(hb, block_id)])
+        lt = np.sort(gmap[hb.tets[:, :4]], axis=1)
+        mine = np.zeros(points.shape[0], bool)
+        mine[own] = True
+        want = _sorted_keys(gt[mine[gt].any(axis=1)])
+        got = _sorted_keys(lt[mine[lt].any(axis=1)])
+        checked += want.shape[0]
+        if want.shape != got.shape or not np.array_equal(want, got):
+            bad_blocks.append(b)
+    return {"closed": not bad_blocks, "bad_blocks": bad_blocks, "tets_checked": int(checked)}
+
+
+def _ghost_globals(hb, block_id):
+    """Global indices of a block's ghosts from their homes (gid, index)."""
+    block_id = np.asarray(block_id)
+    if hb.ghost_gid is None or len(hb.ghost_gid) == 0:
+        return np.zeros(0, np.int64)
+    out = np.zeros(len(hb.ghost_gid), np.int64)
+    for g in np.unique(hb.ghost_gid):
+        idx = np.nonzero(block_id == g)[0]
+        sel = hb.ghost_gid == g
+        out[sel] = idx[hb.ghost_cell[sel]]
+    return out
+
+
+def stars_certified(blocks, points, block_id, workers: int = 1, log=None) -> dict:
+    """A certificate that every original particle's star in its block is its
+    Delaunay star among all points, without tessellating all points: every
+    tet of the star has an empty circumsphere (its 5 nearest points to the
+    circumcenter, in float64, include no point strictly inside beyond a 1e-9
+    relative margin), and every face of the star on the block tessellation's
+    hull is a facet of the convex hull of all points (so the star is closed
+    wherever the global one is).  For points in general position such a star
+    is the Delaunay star."""
+    from scipy.spatial import ConvexHull, cKDTree
+    points = np.ascontiguousarray(points, dtype=np.float32)
+    block_id = np.asarray(block_id)
+    p64 = points.astype(np.float64)
+    tree = cKDTree(p64)
+    hull = {tuple(f) for f in np.sort(ConvexHull(p64).simplices, axis=1).tolist()}
+    bad_blocks, open_faces, nonempty, checked = [], 0, 0, 0
+    for hb in blocks:
+        b = hb.gid
+        own = np.nonzero(block_id == b)[0]
+        gmap = np.concatenate([own, _ghost_globals(hb, block_id)])
+        sel = (hb.tets[:, :4] < hb.num_orig).any(axis=1)
+        tets = hb.tets[sel]
+        tv = tets[:, :4]
+        p = hb.particles.astype(np.float64)
+        a, bb, c, d = (p[tv[:, k]] for k in range(4))
+        # circumcenter of each tet in float64: solve 2 (x - d) . [a-d; b-d; c-d] = |.|^2
+        m = np.stack([a - d, bb - d, c - d], axis=1)
+        rhs = 0.5 * np.einsum("ijk,ijk->ij", m, m)
+        ok = np.abs(np.linalg.det(m)) > 0
+        cen = np.full((tv.shape[0], 3), np.nan)
+        cen[ok] = np.linalg.solve(m[ok], rhs[ok][:, :, None])[:, :, 0] + d[ok]
+        rad = np.linalg.norm(cen - d, axis=1)
+        dist, idx = tree.query(np.where(ok[:, None], cen, 0.0), k=5, workers=workers)
+        gv = gmap[tv]
+        inside = (dist < rad[:, None] * (1.0 - 1e-9)) & ~np.any(idx[:, :, None] == gv[:, None, :],
+                                                                axis=2)
+        n_bad = int(inside[ok].any(axis=1).sum())
+        # hull faces of the block tessellation that touch an original
+        faces = []
+        for j in range(4):
+            rows = tets[:, 4 + j] == -1
+            if rows.any():
+                f = np.delete(tv[rows], j, axis=1)
+                f = f[(f < hb.num_orig).any(axis=1)]
+                faces.append(np.sort(gmap[f], axis=1))
+        n_open = 0
+        if faces:
+            fl = np.unique(np.concatenate(faces), axis=0)
+            n_open = sum(1 for f in fl.tolist() if tuple(f) not in hull)
+        open_faces += n_open
+        nonempty += n_bad
+        checked += int(tv.shape[0])
+        if n_open or n_bad:
+            bad_blocks.append(b)
+        if log:
+            log(f"  block {b}: {tv.shape[0]} tets checked, {n_bad} non-empty spheres, "
+                f"{n_open} hull faces not on the convex hull")
+    return {"closed": not bad_blocks, "bad_blocks": bad_blocks, "nonempty_spheres": nonempty,
+            "open_faces_off_hull": open_faces, "tets_checked": checked}

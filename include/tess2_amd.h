@@ -269,11 +269,13 @@ enum {
     TESS2_OPT_CHUNK_SMALL = 2,    /* cells per work item, blocks with < 20000 cells */
     TESS2_OPT_CHUNK_MEDIUM = 3,   /* cells per work item, blocks with < 100000 cells */
     TESS2_OPT_CHUNK_LARGE = 4,    /* cells per work item, larger blocks */
-    TESS2_OPT_CELL_KERNEL = 5     /* single-block cell kernel: -1 (default) chosen per run
-                                   * (1, or 2 when cell boxes are large: 16 or more grid
-                                   * points per cell); 0 one cell per wave; 1 two cells per
-                                   * wave in the star / face / ring phase; 2 the same with
-                                   * the verdict masks of boxes up to 512 points */
+    TESS2_OPT_CELL_KERNEL = 5     /* cell kernel: -1 (default) chosen per run (one block:
+                                   * 1, or 2 when cell boxes are large, 16 or more grid
+                                   * points per cell; several blocks: 0); 0 one cell per
+                                   * wave; 1 two cells per wave in the star / face / ring
+                                   * phase; 2 the same with the verdict masks of boxes up
+                                   * to 512 points (several blocks: 1 and 2 both run the
+                                   * two-cell persistent kernel with those masks) */
 };
 
 /* Set / read one option.  Chunk sizes 1..4096, waves per CU 1..64, cell

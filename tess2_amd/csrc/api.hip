@@ -942,8 +942,17 @@ static int enqueue_kernels(tess2_ctx *c, const RunArgs &a)
         launch_cells_ref_slow(c->h_ins[0], c->h_gps[0], c->h_outs[0], 1, s);
     } else {
         if (ev) HIPCHK(c, hipEventRecord(c->ev_k0, s));
-        launch_cells_ref_batched(c->d_ins, c->d_gps, c->d_outs, c->d_router, c->d_chunks, c->xq,
-                                 c->d_queue, c->ncu * c->waves_per_cu, 1, s);
+        // one cell per wave unless a two-cell kernel is asked for: on 64
+        // Plummer blocks the two-cell kernel measured 24.1 vs 22.7 ms (its
+        // 48-tet capacity defers 6,241 cells of the dense cores)
+        if (c->cell_kernel <= 0) {
+            launch_cells_ref_batched(c->d_ins, c->d_gps, c->d_outs, c->d_router, c->d_chunks,
+                                     c->xq, c->d_queue, c->ncu * c->waves_per_cu, 1, s);
+        } else {
+            launch_cells_mc_batched(c->d_ins, c->d_gps, c->d_outs, c->d_router, c->d_chunks,
+                                    c->xq, c->d_queue, c->ncu * c->waves_per_cu, 1, s);
+            launch_cells_ref_slow_batched(c->d_ins, c->d_router, 1, s);
+        }
         if (ev) HIPCHK(c, hipEventRecord(c->ev_k1, s));
     }
     // phase C: ordered fold of shared points, statistics, one copy of the

@@ -81,9 +81,15 @@ struct McLayout {
     static constexpr int BYTES = 56 * SG + 8 * HG;
 };
 
-template <int CPW, int SG, int HG, int FG, int RMAX, int MINW, bool WIDE>
-__global__ __launch_bounds__(64, MINW) void k_cells_mc(BlockIn in, GridP gp, CellOut o,
-                                                       int want_density)
+// One wave's CPW cells: v is the cell of this lane's group (-1 for none).
+// Every lane of the wave calls it (it has wave barriers); smem holds CPW
+// group regions of McLayout<SG, HG>::BYTES.  rt / self route off-block
+// points to the other blocks of a multi-block run.
+template <int CPW, int SG, int HG, int FG, int RMAX, bool WIDE>
+__device__ __forceinline__ void cells_mc_body(const int v, const BlockIn &in, const GridP &gp,
+                                              const CellOut &o, const int want_density,
+                                              const Router *rt, const int self,
+                                              unsigned char *smem)
 {
     constexpr int G = 64 / CPW;
     using L = McLayout<SG, HG>;
@@ -92,11 +98,7 @@ __global__ __launch_bounds__(64, MINW) void k_cells_mc(BlockIn in, GridP gp, Cel
     static_assert(RMAX <= SG, "runs fit the neighbour area");
     constexpr int NFG = (FG + G - 1) / G;
     constexpr int LMAX = 4096;
-    __shared__ __align__(16) unsigned char smem[CPW * L::BYTES];
 
-    const int nitems = (in.ncells + CPW - 1) / CPW;
-    const int item = xcd_item((int)blockIdx.x, nitems);
-    if (item < 0) return;
     const int lane = (int)(threadIdx.x & 63);
     const int g = lane / G, gl = lane % G;
     const unsigned long long gbits =
@@ -112,8 +114,6 @@ __global__ __launch_bounds__(64, MINW) void k_cells_mc(BlockIn in, GridP gp, Cel
     int *s_tid = reinterpret_cast<int *>(R + L::TID);
     int *s_aux = reinterpret_cast<int *>(R + L::AUX);
 
-    const int k = item * CPW + g;
-    const int v = k < in.ncells ? (in.order ? in.order[k] : k) : -1;
     const int vt = v >= 0 ? in.v2t[v] : -1;
     int st = v < 0 ? G_NONE : (vt < 0 ? G_HULL : G_RUN);
 
@@ -444,9 +444,58 @@ __global__ __launch_bounds__(64, MINW) void k_cells_mc(BlockIn in, GridP gp, Cel
         scan_deposit<1, RMAX, WIDE>(vq, siteq, Fq, nx1, ny1, nz1, fx1, fy1, fz1,
                                      rdl_f(mnx, ql), rdl_f(mny, ql), rdl_f(mnz, ql),
                                      rdl_f(mxx, ql), rdl_f(mxy, ql), rdl_f(mxz, ql), gp, o,
-                                     nullptr, 0, const_cast<float4 *>(faq),
+                                     rt, self, const_cast<float4 *>(faq),
                                      const_cast<float2 *>(fbq), true, runs);
         __syncthreads();
+    }
+}
+
+template <int CPW, int SG, int HG, int FG, int RMAX, int MINW, bool WIDE>
+__global__ __launch_bounds__(64, MINW) void k_cells_mc(BlockIn in, GridP gp, CellOut o,
+                                                       int want_density)
+{
+    __shared__ __align__(16) unsigned char smem[CPW * McLayout<SG, HG>::BYTES];
+    const int nitems = (in.ncells + CPW - 1) / CPW;
+    const int item = xcd_item((int)blockIdx.x, nitems);
+    if (item < 0) return;
+    const int k = item * CPW + (int)(threadIdx.x & 63) / (64 / CPW);
+    const int v = k < in.ncells ? (in.order ? in.order[k] : k) : -1;
+    cells_mc_body<CPW, SG, HG, FG, RMAX, WIDE>(v, in, gp, o, want_density, nullptr, 0, smem);
+}
+
+// All blocks of a run in one persistent launch (as k_cells_ref_batched):
+// waves take chunks of consecutive cells of one block from their XCD's
+// queue, then from the others, and run CPW cells of the chunk at a time.
+template <int CPW, int SG, int HG, int FG, int RMAX, int MINW, bool WIDE>
+__global__ __launch_bounds__(64, MINW) void k_cells_mc_batched(
+    const BlockIn *__restrict__ ins, const GridP *__restrict__ gps,
+    const CellOut *__restrict__ outs, const Router *__restrict__ rt,
+    const Chunk *__restrict__ chunks, XcdQueues xq, unsigned *queue, int want_density)
+{
+    __shared__ __align__(16) unsigned char smem[CPW * McLayout<SG, HG>::BYTES];
+    const int lane = (int)(threadIdx.x & 63);
+    const int g = lane / (64 / CPW);
+    const int x0 = (int)(blockIdx.x & 7);
+    for (int q = 0; q < 8;) {
+        const int x = (x0 + q) & 7;
+        int c = 0;
+        if (lane == 0) c = (int)atomicAdd(&queue[x], 1u);
+        c = xq.start[x] + __builtin_amdgcn_readfirstlane(c);
+        if (c >= xq.start[x + 1]) {   // range x is done: help the next one
+            ++q;
+            continue;
+        }
+        const Chunk ch = chunks[c];
+        const BlockIn in = ins[ch.block];
+        const GridP gp = gps[ch.block];
+        const CellOut o = outs[ch.block];
+        for (int j = 0; j < ch.count; j += CPW) {
+            const int kk = ch.start + j + g;
+            const int v = j + g < ch.count ? (in.order ? in.order[kk] : kk) : -1;
+            cells_mc_body<CPW, SG, HG, FG, RMAX, WIDE>(v, in, gp, o, want_density, rt, ch.block,
+                                                       smem);
+            __syncthreads();
+        }
     }
 }
 
@@ -457,6 +506,16 @@ static void launch_mc(const BlockIn &in, const GridP &gp, const CellOut &o, int 
     const int nitems = (in.ncells + CPW - 1) / CPW;
     hipLaunchKernelGGL((k_cells_mc<CPW, SG, HG, FG, RMAX, MINW, WIDE>), dim3(xcd_grid(nitems)),
                        dim3(64), 0, s, in, gp, o, want_density);
+}
+
+void launch_cells_mc_batched(const BlockIn *ins, const GridP *gps, const CellOut *outs,
+                             const Router *rt, const Chunk *chunks, const XcdQueues &xq,
+                             unsigned *queue, int waves, int want_density, hipStream_t s)
+{
+    launch_fill_u32(queue, 8, 0u, s);
+    if (xq.start[8] > 0)
+        hipLaunchKernelGGL((k_cells_mc_batched<2, 48, 64, 48, 48, 6, true>), dim3(waves), dim3(64),
+                           0, s, ins, gps, outs, rt, chunks, xq, queue, want_density);
 }
 
 int launch_cells_mc(const BlockIn &in, const GridP &gp, const CellOut &o, int want_density,

@@ -517,6 +517,12 @@ void launch_cells_ref_batched(const BlockIn *ins, const GridP *gps, const CellOu
     if (xq.start[8] > 0)
         hipLaunchKernelGGL((k_cells_ref_batched<1, 1, 64>), dim3(waves), dim3(64), 0, s, ins, gps,
                            outs, rt, chunks, xq, queue, want_density);
+    launch_cells_ref_slow_batched(ins, rt, want_density, s);
+}
+
+void launch_cells_ref_slow_batched(const BlockIn *ins, const Router *rt, int want_density,
+                                   hipStream_t s)
+{
     hipLaunchKernelGGL((k_cells_ref_slow_batched<8, 2, 512>), dim3(1024), dim3(64), 0, s, ins, rt,
                        want_density);
 }

@@ -193,6 +193,14 @@ void launch_cells_ref_slow(const BlockIn &in, const GridP &gp, const CellOut &o,
 // picks cells per wave and capacities; returns -1 for an unknown variant
 int launch_cells_mc(const BlockIn &in, const GridP &gp, const CellOut &o, int want_density,
                     int variant, hipStream_t s);
+// large-capacity pass over the cells every block's fast pass deferred
+void launch_cells_ref_slow_batched(const BlockIn *ins, const Router *rt, int want_density,
+                                   hipStream_t s);
+// the two-cell kernel over all blocks in one persistent launch (the deferred
+// cells then need launch_cells_ref_batched's large-capacity pass)
+void launch_cells_mc_batched(const BlockIn *ins, const GridP *gps, const CellOut *outs,
+                             const Router *rt, const Chunk *chunks, const XcdQueues &xq,
+                             unsigned *queue, int waves, int want_density, hipStream_t s);
 void launch_cic(const BlockIn &in, const GridP &gp, const CellOut &o, const Router *rt_dev,
                 int self, hipStream_t s);
 void launch_cells_ref_batched(const BlockIn *ins, const GridP *gps, const CellOut *outs,

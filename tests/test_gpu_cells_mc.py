@@ -70,3 +70,29 @@ def test_cells_mc_bitexact(oracle, blocks, name, g, variant):
     assert np.array_equal(_bits(out["volumes"]), _bits(oracle.volumes(ob)))
     if name == "hub":
         assert st["cells_deferred"] >= 1
+
+
+@pytest.mark.parametrize("name", ["uniform8", "sparse8", "plummer8"])
+def test_cells_mc_batched_bitexact(oracle, name):
+    """The two-cell persistent kernel over several blocks (option 1), against
+    the oracle's multi-block model: per-block statuses, counts, densities."""
+    from .test_gpu_multiblock import _uniform_blocks
+    say("cells_mc batched", name)
+    if name == "uniform8":
+        blocks = _uniform_blocks(2, 1200, 51, 0.1)
+    elif name == "sparse8":
+        blocks = _uniform_blocks(2, 150, 53, 0.2)
+    else:
+        pts, bid, counts, bounds = synthetic.plummer_blocks(2, 500, 4000, seed=52)
+        blocks = ingest.decomposed_blocks(pts, bid, bounds, 0.05)
+    g = (48, 48, 48)
+    run = D.StagedRun(blocks)
+    run.set_option("cell_kernel", 1)
+    run.run(g)
+    run.sync()
+    want = oracle.dense_multi([oracle.Block.from_host(b) for b in blocks], g)
+    for i in range(len(blocks)):
+        s, npts = run.cells(i)
+        assert np.array_equal(s, want["status"][i])
+        assert np.array_equal(npts, want["npts"][i])
+        assert np.array_equal(_bits(run.fetch(i)["density"]), _bits(want["density"][i])), i

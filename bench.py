@@ -305,6 +305,9 @@ def main():
                     help="multi-block workloads: also build the ghost layers by the exchange "
                          "rounds with the GPU's incomplete_cells test and check that every "
                          "original star closes")
+    ap.add_argument("--calibrate", action="store_true",
+                    help="run a plain copy kernel (256 MiB read, 256 MiB written) three times "
+                         "and exit: the reference the HBM traffic counters are scaled by")
     ap.add_argument("--no-graph", action="store_true",
                     help="time eager launches instead of graph replays (for tracing tools)")
     ap.add_argument("--save-block", default=None,
@@ -316,6 +319,16 @@ def main():
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); this build has no "
               f"multi-GPU path", file=sys.stderr)
         sys.exit(2)
+
+    if args.calibrate:
+        import torch
+        a = torch.ones(64 << 20, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            torch.mul(a, 1.0, out=b)     # one elementwise kernel: 4 B read + 4 B written per item
+        torch.cuda.synchronize()
+        log(f"[calibrate] copied {a.numel() * 4} bytes x3")
+        return
 
     from tess2_amd import host
 

@@ -271,15 +271,18 @@ enum {
     TESS2_OPT_CHUNK_LARGE = 4,    /* cells per work item, larger blocks */
     TESS2_OPT_CELL_KERNEL = 5     /* cell kernel: -1 (default) chosen per run (one block:
                                    * 1, or 2 when cell boxes are large, 16 or more grid
-                                   * points per cell; several blocks: 0); 0 one cell per
+                                   * points per cell; several blocks: 3); 0 one cell per
                                    * wave; 1 two cells per wave in the star / face / ring
                                    * phase; 2 the same with the verdict masks of boxes up
                                    * to 512 points (several blocks: 1 and 2 both run the
-                                   * two-cell persistent kernel with those masks) */
+                                   * two-cell persistent kernel with those masks); 3 one
+                                   * cell per wave with the star search's record loads
+                                   * issued before its table work (several blocks only;
+                                   * one block runs 0) */
 };
 
 /* Set / read one option.  Chunk sizes 1..4096, waves per CU 1..64, cell
- * kernel -1..2. */
+ * kernel -1..3. */
 int tess2_set_option(tess2_ctx *ctx, int option, int value);
 int tess2_get_option(tess2_ctx *ctx, int option, int *value);
 

@@ -172,6 +172,7 @@ static bool same_args(const RunArgs &x, const RunArgs &y)
 // at 8 points per cell the masks cost 2 %).
 static int cell_kernel_for(const tess2_ctx *c, int ncells, unsigned long long ngrid)
 {
+    if (c->cell_kernel == 3) return 0;   // a multi-block choice; one block: one cell per wave
     if (c->cell_kernel >= 0) return c->cell_kernel;
     return ngrid >= 16ull * (unsigned long long)std::max(ncells, 1) ? 2 : 1;
 }
@@ -942,12 +943,16 @@ static int enqueue_kernels(tess2_ctx *c, const RunArgs &a)
         launch_cells_ref_slow(c->h_ins[0], c->h_gps[0], c->h_outs[0], 1, s);
     } else {
         if (ev) HIPCHK(c, hipEventRecord(c->ev_k0, s));
-        // one cell per wave unless a two-cell kernel is asked for: on 64
-        // Plummer blocks the two-cell kernel measured 24.1 vs 22.7 ms (its
-        // 48-tet capacity defers 6,241 cells of the dense cores)
-        if (c->cell_kernel <= 0) {
+        // one cell per wave, the star search's record loads issued before
+        // the table work (option 0: after it, in a gather round per level),
+        // unless a two-cell kernel is asked for: on 64 Plummer blocks the
+        // two-cell kernel measured 24.1 vs 22.7 ms (its 48-tet capacity
+        // defers 6,241 cells of the dense cores) and the early loads 23.6 vs
+        // 24.0 ms
+        if (c->cell_kernel <= 0 || c->cell_kernel == 3) {
             launch_cells_ref_batched(c->d_ins, c->d_gps, c->d_outs, c->d_router, c->d_chunks,
-                                     c->xq, c->d_queue, c->ncu * c->waves_per_cu, 1, s);
+                                     c->xq, c->d_queue, c->ncu * c->waves_per_cu, 1, s,
+                                     c->cell_kernel != 0);
         } else {
             launch_cells_mc_batched(c->d_ins, c->d_gps, c->d_outs, c->d_router, c->d_chunks,
                                     c->xq, c->d_queue, c->ncu * c->waves_per_cu, 1, s);
@@ -1149,7 +1154,7 @@ extern "C" int tess2_set_option(tess2_ctx *c, int option, int value)
         c->waves_per_cu = value;
         return 0;
     case TESS2_OPT_CELL_KERNEL:
-        if (value < -1 || value > 2) return fail(c, TESS2_E_ARG, "cell kernel: -1..2");
+        if (value < -1 || value > 3) return fail(c, TESS2_E_ARG, "cell kernel: -1..3");
         if (value != c->cell_kernel) c->gen++;   // a recorded graph is recorded again
         c->cell_kernel = value;
         return 0;

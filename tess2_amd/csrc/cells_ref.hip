@@ -79,7 +79,7 @@ __device__ __forceinline__ int comp(const int4 &q, int i)
 template <int NS> constexpr int hash_entries() { return 2 * 64 * NS; }
 template <int NS> constexpr int hash_ints() { return 2 * hash_entries<NS>() + 2 * 64 * NS; }
 
-template <int NS, int NF, int RMAX, bool WIDE = false>
+template <int NS, int NF, int RMAX, bool WIDE = false, bool SPEC = false>
 __device__ void cell_ref_body(const int v, const BlockIn &in, const GridP &gp,
                               const CellOut &o, const int want_density, const bool can_defer,
                               int4 *s_v, int *s_ns, float4 *s_cc, int4 *s_u, int *s_h,
@@ -130,6 +130,8 @@ __device__ void cell_ref_body(const int v, const BlockIn &in, const GridP &gp,
             const int key = base + lane;
             int cand = -1, s = 0, i = 0;
             bool valid = false, h = false;
+            int4 rv = make_int4(0, 0, 0, 0), rn = make_int4(0, 0, 0, 0);
+            float4 rc = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
             if (key < ncand) {
                 s = f0 + (key >> 2);
                 i = key & 3;
@@ -140,6 +142,11 @@ __device__ void cell_ref_body(const int v, const BlockIn &in, const GridP &gp,
                     else {
                         valid = true;
                         cand = ni;
+                        if (SPEC) {   // the record's loads overlap the table work below
+                            rv = in.tets[2 * ni];
+                            rn = in.tets[2 * ni + 1];
+                            rc = in.cc[ni];
+                        }
                     }
                 }
             }
@@ -158,7 +165,14 @@ __device__ void cell_ref_body(const int v, const BlockIn &in, const GridP &gp,
             if (lead) {
                 const int slot = S + __popcll(L & ((1ull << lane) - 1ull));
                 hst[e] = -(slot + 1);
-                if (slot < SMAX) s_tid[slot] = cand;
+                if (slot < SMAX) {
+                    s_tid[slot] = cand;
+                    if (SPEC) {
+                        s_v[slot] = rv;
+                        s_u[slot] = rn;
+                        s_cc[slot] = rc;
+                    }
+                }
             }
             S += __popcll(L);
             if (key < ncand) s_ns[s * 4 + i] = valid ? -hst[e] - 1 : -1;
@@ -169,14 +183,17 @@ __device__ void cell_ref_body(const int v, const BlockIn &in, const GridP &gp,
         }
         if (hull || over) break;
         __syncthreads();
-        // gather the records of the new slots [f1, S)
-        for (int b = f1; b < S; b += 64) {
-            const int s = b + lane;
-            if (s < S) {
-                const int id = s_tid[s];
-                s_v[s] = in.tets[2 * id];
-                s_u[s] = in.tets[2 * id + 1];
-                s_cc[s] = in.cc[id];
+        // gather the records of the new slots [f1, S) (already written by
+        // the winning lanes when SPEC)
+        if (!SPEC) {
+            for (int b = f1; b < S; b += 64) {
+                const int s = b + lane;
+                if (s < S) {
+                    const int id = s_tid[s];
+                    s_v[s] = in.tets[2 * id];
+                    s_u[s] = in.tets[2 * id + 1];
+                    s_cc[s] = in.cc[id];
+                }
             }
         }
         __syncthreads();
@@ -446,7 +463,7 @@ __global__ __launch_bounds__(64) void k_cells_ref_slow(BlockIn in, GridP gp, Cel
 // until all are empty.  Chunk sizes by block size class (tess2_autotune).  At least 6
 // waves per SIMD: the per-block descriptors read from memory otherwise push
 // the register count down to 4 waves (30 vs 34.5 ms on 64 Plummer blocks).
-template <int NS, int NF, int RMAX>
+template <int NS, int NF, int RMAX, bool SPEC>
 __global__ __launch_bounds__(64, 6) void k_cells_ref_batched(const BlockIn *__restrict__ ins,
                                                           const GridP *__restrict__ gps,
                                                           const CellOut *__restrict__ outs,
@@ -479,7 +496,8 @@ __global__ __launch_bounds__(64, 6) void k_cells_ref_batched(const BlockIn *__re
         for (int j = 0; j < ch.count; ++j) {
             const int k = ch.start + j;
             const int v = in.order ? in.order[k] : k;
-            cell_ref_body<NS, NF, RMAX, true>(v, in, gp, o, want_density, true, s_v, s_ns, s_cc, s_u,
+            cell_ref_body<NS, NF, RMAX, true, SPEC>(v, in, gp, o, want_density, true, s_v, s_ns,
+                                                    s_cc, s_u,
                                         s_h, rt, ch.block);
             __syncthreads();
         }
@@ -511,12 +529,18 @@ __global__ __launch_bounds__(64) void k_cells_ref_slow_batched(const BlockIn *__
 
 void launch_cells_ref_batched(const BlockIn *ins, const GridP *gps, const CellOut *outs,
                               const Router *rt, const Chunk *chunks, const XcdQueues &xq,
-                              unsigned *queue, int waves, int want_density, hipStream_t s)
+                              unsigned *queue, int waves, int want_density, hipStream_t s,
+                              bool spec)
 {
     launch_fill_u32(queue, 8, 0u, s);
-    if (xq.start[8] > 0)
-        hipLaunchKernelGGL((k_cells_ref_batched<1, 1, 64>), dim3(waves), dim3(64), 0, s, ins, gps,
-                           outs, rt, chunks, xq, queue, want_density);
+    if (xq.start[8] > 0) {
+        if (spec)
+            hipLaunchKernelGGL((k_cells_ref_batched<1, 1, 64, true>), dim3(waves), dim3(64), 0, s,
+                               ins, gps, outs, rt, chunks, xq, queue, want_density);
+        else
+            hipLaunchKernelGGL((k_cells_ref_batched<1, 1, 64, false>), dim3(waves), dim3(64), 0, s,
+                               ins, gps, outs, rt, chunks, xq, queue, want_density);
+    }
     launch_cells_ref_slow_batched(ins, rt, want_density, s);
 }
 

@@ -203,9 +203,12 @@ void launch_cells_mc_batched(const BlockIn *ins, const GridP *gps, const CellOut
                              unsigned *queue, int waves, int want_density, hipStream_t s);
 void launch_cic(const BlockIn &in, const GridP &gp, const CellOut &o, const Router *rt_dev,
                 int self, hipStream_t s);
+// spec: each star-search candidate loads its tet's record before the table
+// decides whether it is new (the loads overlap the LDS work)
 void launch_cells_ref_batched(const BlockIn *ins, const GridP *gps, const CellOut *outs,
                               const Router *rt, const Chunk *chunks, const XcdQueues &xq,
-                              unsigned *queue, int waves, int want_density, hipStream_t s);
+                              unsigned *queue, int waves, int want_density, hipStream_t s,
+                              bool spec);
 // ordered fold of shared grid points and per-block statistics for nb blocks
 // at once (outs/gps: device arrays; the capacities size the grids)
 void launch_fold(const CellOut *outs, const GridP *gps, int nb, unsigned max_ovf_cap,

@@ -72,10 +72,13 @@ def test_cells_mc_bitexact(oracle, blocks, name, g, variant):
         assert st["cells_deferred"] >= 1
 
 
+@pytest.mark.parametrize("variant", [1, 3])
 @pytest.mark.parametrize("name", ["uniform8", "sparse8", "plummer8"])
-def test_cells_mc_batched_bitexact(oracle, name):
-    """The two-cell persistent kernel over several blocks (option 1), against
-    the oracle's multi-block model: per-block statuses, counts, densities."""
+def test_cells_mc_batched_bitexact(oracle, name, variant):
+    """The persistent kernels over several blocks other than the default (1:
+    two cells per wave; 3: one cell per wave with the star search's record
+    loads issued before the table work), against the oracle's multi-block
+    model: per-block statuses, counts, densities."""
     from .test_gpu_multiblock import _uniform_blocks
     say("cells_mc batched", name)
     if name == "uniform8":
@@ -87,7 +90,7 @@ def test_cells_mc_batched_bitexact(oracle, name):
         blocks = ingest.decomposed_blocks(pts, bid, bounds, 0.05)
     g = (48, 48, 48)
     run = D.StagedRun(blocks)
-    run.set_option("cell_kernel", 1)
+    run.set_option("cell_kernel", variant)
     run.run(g)
     run.sync()
     want = oracle.dense_multi([oracle.Block.from_host(b) for b in blocks], g)

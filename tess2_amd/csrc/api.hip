@@ -453,8 +453,7 @@ static int validate(tess2_ctx *c, const tess2_block_t &b)
 }
 
 // Work list of the batched launch: each block's cells (in its Morton order)
-// cut into chunks whose size depends on the block's size class; blocks in
-// decreasing size so that the big ones start first.
+// cut into chunks whose size depends on the block's size class.
 static int build_chunks(tess2_ctx *c)
 {
     // blocks in staging (gid) order, each block's cells in Morton order, so

@@ -85,7 +85,7 @@ struct McLayout {
 // Every lane of the wave calls it (it has wave barriers); smem holds CPW
 // group regions of McLayout<SG, HG>::BYTES.  rt / self route off-block
 // points to the other blocks of a multi-block run.
-template <int CPW, int SG, int HG, int FG, int RMAX, bool WIDE>
+template <int CPW, int SG, int HG, int FG, int RMAX, bool WIDE, bool LINE>
 __device__ __forceinline__ void cells_mc_body(const int v, const BlockIn &in, const GridP &gp,
                                               const CellOut &o, const int want_density,
                                               const Router *rt, const int self,
@@ -441,7 +441,7 @@ __device__ __forceinline__ void cells_mc_body(const int v, const BlockIn &in, co
             fz1[0] = b.y;
         }
         const float4 siteq = in.pts[vq];
-        scan_deposit<1, RMAX, WIDE>(vq, siteq, Fq, nx1, ny1, nz1, fx1, fy1, fz1,
+        scan_deposit<1, RMAX, WIDE, LINE>(vq, siteq, Fq, nx1, ny1, nz1, fx1, fy1, fz1,
                                      rdl_f(mnx, ql), rdl_f(mny, ql), rdl_f(mnz, ql),
                                      rdl_f(mxx, ql), rdl_f(mxy, ql), rdl_f(mxz, ql), gp, o,
                                      rt, self, const_cast<float4 *>(faq),
@@ -460,7 +460,8 @@ __global__ __launch_bounds__(64, MINW) void k_cells_mc(BlockIn in, GridP gp, Cel
     if (item < 0) return;
     const int k = item * CPW + (int)(threadIdx.x & 63) / (64 / CPW);
     const int v = k < in.ncells ? (in.order ? in.order[k] : k) : -1;
-    cells_mc_body<CPW, SG, HG, FG, RMAX, WIDE>(v, in, gp, o, want_density, nullptr, 0, smem);
+    cells_mc_body<CPW, SG, HG, FG, RMAX, WIDE, false>(v, in, gp, o, want_density, nullptr, 0,
+                                                      smem);
 }
 
 // All blocks of a run in one persistent launch (as k_cells_ref_batched):
@@ -492,8 +493,8 @@ __global__ __launch_bounds__(64, MINW) void k_cells_mc_batched(
         for (int j = 0; j < ch.count; j += CPW) {
             const int kk = ch.start + j + g;
             const int v = j + g < ch.count ? (in.order ? in.order[kk] : kk) : -1;
-            cells_mc_body<CPW, SG, HG, FG, RMAX, WIDE>(v, in, gp, o, want_density, rt, ch.block,
-                                                       smem);
+            cells_mc_body<CPW, SG, HG, FG, RMAX, WIDE, true>(v, in, gp, o, want_density, rt,
+                                                             ch.block, smem);
             __syncthreads();
         }
     }

@@ -402,7 +402,7 @@ __device__ void cell_ref_body(const int v, const BlockIn &in, const GridP &gp,
             if (isnan(c00.y)) { mny = c00.y; mxy = c00.y; }
             if (isnan(c00.z)) { mnz = c00.z; mxz = c00.z; }
         }
-        scan_deposit<NF, RMAX, WIDE>(v, site, F, fnx, fny, fnz, ffx, ffy, ffz, mnx, mny, mnz,
+        scan_deposit<NF, RMAX, WIDE, WIDE>(v, site, F, fnx, fny, fnz, ffx, ffy, ffz, mnx, mny, mnz,
                                      mxx, mxy, mxz, gp, o, rt, self, s_cc,
                                      reinterpret_cast<float2 *>(s_v), false, s_u);
         return;

@@ -7,7 +7,7 @@
 namespace t2 {
 
 // The reference's two x walks along one scan line (src/dense.cpp:1546-1620)
-// from the line's verdict bits (bit x = PtInCell at x, n0 <= 32 points): each
+// from the line's verdict bits (bit x = PtInCell at x, n0 <= 64 points): each
 // walk moves one point at a time until the verdict changes, so it ends at the
 // nearest bit of the other value, found here with one bit scan.
 //   left walk from x0 = x_left: inside -> down through the run (x_left = the
@@ -19,7 +19,7 @@ namespace t2 {
 __host__ __device__ __forceinline__ void walk_row_bits(unsigned long long row, int n0, int &x_left,
                                               int &x_right, int &min_xi, int &max_xi)
 {
-    const unsigned long long all = (1ull << n0) - 1ull;
+    const unsigned long long all = n0 >= 64 ? ~0ull : (1ull << n0) - 1ull;
     const int x0 = x_left, xr = x_right;
     const bool x_in = (row >> x0) & 1ull;
     if (x_in) {

@@ -18,11 +18,33 @@ enum { ST_SCAN = 0, ST_CIC = 1, ST_INCOMPLETE = 2, ST_OUTSIDE = 3, ST_DEGENERATE
 
 __device__ __forceinline__ bool wide_bit(unsigned long long w, int b) { return (w >> b) & 1ull; }
 
+// PtInCell at (qx, qy, qz) against the F face planes staged in LDS: the
+// largest and smallest significant distance decide it (a face with d >= 0
+// exists iff hi >= 0, one with d < 0 iff lo < 0), the same float operations
+// as a face-parallel probe's
+__device__ __forceinline__ bool lds_verdict(float qx, float qy, float qz, int F,
+                                            const float4 *s_fa, const float2 *s_fb, float eps)
+{
+    float hi = -INFINITY, lo = INFINITY;
+#pragma unroll 4
+    for (int k = 0; k < F; ++k) {
+        const float4 a = s_fa[k];
+        const float2 b = s_fb[k];
+        const float d = a.x * (qx - a.w) + a.y * (qy - b.x) + a.z * (qz - b.y);
+        const bool sig = fabsf(d) > eps;
+        hi = fmaxf(hi, sig ? d : -INFINITY);
+        lo = fminf(lo, sig ? d : INFINITY);
+    }
+    return !(hi >= 0.0f && lo < 0.0f);
+}
+
 // v: the cell; site: its particle; F faces with planes (fn*, ff*) per lane;
 // mn*/mx*: the cell bbox, wave-uniform (the reference's fold, NaN included);
 // s_fa / s_fb: LDS room for F face planes {n, f0.x} / {f0.y, f0.z}, already
-// filled when faces_staged; runs: LDS room for RMAX scan runs.
-template <int NF, int RMAX, bool WIDE>
+// filled when faces_staged; runs: LDS room for RMAX scan runs.  WIDE adds
+// the verdict masks of boxes up to 512 points, LINE the line windows of
+// larger boxes (each costs registers: on only where such boxes are common).
+template <int NF, int RMAX, bool WIDE, bool LINE>
 __device__ __forceinline__ void scan_deposit(
     const int v, const float4 site, const int F, const float (&fnx)[NF], const float (&fny)[NF],
     const float (&fnz)[NF], const float (&ffx)[NF], const float (&ffy)[NF],
@@ -68,7 +90,8 @@ __device__ __forceinline__ void scan_deposit(
     // ballotted into one wave-uniform mask.  A verdict is the same function
     // of the same float operations as a face-parallel probe's; the scan
     // then walks each line with bit scans and reads single bits where it
-    // probes.  Larger boxes probe face-parallel as they go.
+    // probes.  Larger boxes: the wide and line-window masks below (WIDE /
+    // LINE kernels), else face-parallel probes as the walk goes.
     const int lb0 = n0 > 1 ? 32 - __clz(n0 - 1) : 0;
     const int lb1 = n1 > 1 ? 32 - __clz(n1 - 1) : 0;
     const int lb2 = n2 > 1 ? 32 - __clz(n2 - 1) : 0;
@@ -80,10 +103,16 @@ __device__ __forceinline__ void scan_deposit(
     // hardly any)
     const int nbox = n0 * n1 * n2;
     const bool use_wide = WIDE && !use_mask && n0 <= 32 && nbox <= 512;
+    // Any larger box with lines of at most 64 points (LINE kernels): the
+    // verdicts of a window of 64 >> lb0 lines of one z plane at once (lane
+    // x + (line << lb0)) in one mask word, the next window when the walk
+    // leaves it.  An empty line then costs one window share instead of the
+    // ~2 n0 face-parallel probes of the two x walks.
+    const bool use_line = LINE && !use_mask && !use_wide && n0 <= 64;
     unsigned long long mk = 0ull;
     unsigned long long w1 = 0ull, w2 = 0ull, w3 = 0ull, w4 = 0ull, w5 = 0ull, w6 = 0ull,
                        w7 = 0ull;
-    if ((use_mask || use_wide) && !faces_staged) {
+    if ((use_mask || use_wide || use_line) && !faces_staged) {
         __syncthreads();
 #pragma unroll
         for (int j = 0; j < NF; ++j) {
@@ -111,17 +140,8 @@ __device__ __forceinline__ void scan_deposit(
             const float qx = p0 + (float)x * gp.step[0];
             const float qy = p1 + (float)y * gp.step[1];
             const float qz = p2 + (float)z * gp.step[2];
-            float hi = -INFINITY, lo = INFINITY;
-#pragma unroll 4
-            for (int k = 0; k < F; ++k) {
-                const float4 a = s_fa[k];
-                const float2 b = s_fb[k];
-                const float d = a.x * (qx - a.w) + a.y * (qy - b.x) + a.z * (qz - b.y);
-                const bool sig = fabsf(d) > eps;
-                hi = fmaxf(hi, sig ? d : -INFINITY);
-                lo = fminf(lo, sig ? d : INFINITY);
-            }
-            const unsigned long long m = __ballot(p < nbox && !(hi >= 0.0f && lo < 0.0f));
+            const unsigned long long m =
+                __ballot(p < nbox && lds_verdict(qx, qy, qz, F, s_fa, s_fb, eps));
             if (c == 0) mk = m;
             else if (c == 1) w1 = m;
             else if (c == 2) w2 = m;
@@ -138,19 +158,7 @@ __device__ __forceinline__ void scan_deposit(
         const float qx = p0 + (float)x * gp.step[0];
         const float qy = p1 + (float)y * gp.step[1];
         const float qz = p2 + (float)z * gp.step[2];
-        // largest and smallest significant distance: a face with d >= 0
-        // exists iff hi >= 0, one with d < 0 iff lo < 0
-        float hi = -INFINITY, lo = INFINITY;
-#pragma unroll 4
-        for (int k = 0; k < F; ++k) {
-            const float4 a = s_fa[k];
-            const float2 b = s_fb[k];
-            const float d = a.x * (qx - a.w) + a.y * (qy - b.x) + a.z * (qz - b.y);
-            const bool sig = fabsf(d) > eps;
-            hi = fmaxf(hi, sig ? d : -INFINITY);
-            lo = fminf(lo, sig ? d : INFINITY);
-        }
-        const bool in_cell = !(hi >= 0.0f && lo < 0.0f);
+        const bool in_cell = lds_verdict(qx, qy, qz, F, s_fa, s_fb, eps);
         mk = __ballot(x < n0 && y < n1 && z < n2 && in_cell);
     }
 
@@ -188,6 +196,9 @@ __device__ __forceinline__ void scan_deposit(
     // found, now that mass/N is known.
     int nruns = 0, tot = 0;
     float mpp = 0.0f;
+    int lz = -1, ly0 = 0;                 // the line window: plane, first line
+    unsigned long long lw = 0ull;         // and its verdicts
+    const int lwin = 64 >> lb0;           // lines per window (lb0 <= 6 there)
     for (int pass = 0; pass < 2; ++pass) {
         if (pass == 1) {
             if (nruns <= RMAX || tot == 0) break;
@@ -208,6 +219,19 @@ __device__ __forceinline__ void scan_deposit(
                     unsigned long long row = WORD(k) >> off;
                     if (off + n0 > 64) row |= WORD(k + 1) << (64 - off);
                     row &= (1ull << n0) - 1ull;
+                    walk_row_bits(row, n0, x_left, x_right, min_xi, max_xi);
+                } else if (use_line) {
+                    if (zi != lz || yi < ly0 || yi >= ly0 + lwin) {
+                        lz = zi;
+                        ly0 = yi;
+                        const int x = lane & ((1 << lb0) - 1), y = yi + (lane >> lb0);
+                        const bool in_cell =
+                            lds_verdict(p0 + (float)x * gp.step[0], p1 + (float)y * gp.step[1],
+                                        p2 + (float)zi * gp.step[2], F, s_fa, s_fb, eps);
+                        lw = __ballot(x < n0 && y < n1 && in_cell);
+                    }
+                    unsigned long long row = lw >> ((yi - ly0) << lb0);
+                    if (n0 < 64) row &= (1ull << n0) - 1ull;
                     walk_row_bits(row, n0, x_left, x_right, min_xi, max_xi);
                 } else {
                 // the left walk starts where x_in was taken: its first

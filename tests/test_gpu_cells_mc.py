@@ -72,28 +72,45 @@ def test_cells_mc_bitexact(oracle, blocks, name, g, variant):
         assert st["cells_deferred"] >= 1
 
 
-@pytest.mark.parametrize("variant", [1, 3])
-@pytest.mark.parametrize("name", ["uniform8", "sparse8", "plummer8"])
-def test_cells_mc_batched_bitexact(oracle, name, variant):
-    """The persistent kernels over several blocks other than the default (1:
-    two cells per wave; 3: one cell per wave with the star search's record
-    loads issued before the table work), against the oracle's multi-block
-    model: per-block statuses, counts, densities."""
+_BATCHED = {}
+
+
+def _batched_blocks(name):
     from .test_gpu_multiblock import _uniform_blocks
-    say("cells_mc batched", name)
-    if name == "uniform8":
-        blocks = _uniform_blocks(2, 1200, 51, 0.1)
-    elif name == "sparse8":
-        blocks = _uniform_blocks(2, 150, 53, 0.2)
-    else:
-        pts, bid, counts, bounds = synthetic.plummer_blocks(2, 500, 4000, seed=52)
-        blocks = ingest.decomposed_blocks(pts, bid, bounds, 0.05)
-    g = (48, 48, 48)
+    if name not in _BATCHED:
+        if name == "uniform8":
+            _BATCHED[name] = _uniform_blocks(2, 1200, 51, 0.1)
+        elif name == "sparse8":
+            _BATCHED[name] = _uniform_blocks(2, 150, 53, 0.2)
+        elif name == "coarse8":
+            _BATCHED[name] = _uniform_blocks(2, 40, 54, 0.3)
+        else:
+            pts, bid, counts, bounds = synthetic.plummer_blocks(2, 500, 4000, seed=52)
+            _BATCHED[name] = ingest.decomposed_blocks(pts, bid, bounds, 0.05)
+    return _BATCHED[name]
+
+
+@pytest.mark.parametrize("variant", [0, 1, 3])
+@pytest.mark.parametrize("name,g", [("uniform8", 48), ("sparse8", 48), ("sparse8", 72),
+                                    ("plummer8", 48), ("coarse8", 200)])
+def test_cells_mc_batched_bitexact(oracle, name, g, variant):
+    """The persistent kernels over several blocks (0: one cell per wave; 1:
+    two cells per wave; 3, the default: one cell per wave with the star
+    search's record loads issued before the table work), against the
+    oracle's multi-block model: per-block statuses, counts, densities.  The
+    sparse cases put cell boxes past the 512-point masks onto the line
+    windows: lines of up to 16 points at 72^3, of 17-64 points (and one
+    longer, on face-parallel probes) in coarse8 at 200^3."""
+    blocks = _batched_blocks(name)
+    say("cells_mc batched", name, g, "variant", variant)
+    g = (g, g, g)
     run = D.StagedRun(blocks)
     run.set_option("cell_kernel", variant)
     run.run(g)
     run.sync()
-    want = oracle.dense_multi([oracle.Block.from_host(b) for b in blocks], g)
+    if (name, g) not in _BATCHED:
+        _BATCHED[name, g] = oracle.dense_multi([oracle.Block.from_host(b) for b in blocks], g)
+    want = _BATCHED[name, g]
     for i in range(len(blocks)):
         s, npts = run.cells(i)
         assert np.array_equal(s, want["status"][i])

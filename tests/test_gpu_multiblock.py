@@ -37,8 +37,11 @@ def multiblocks():
     out = {}
     out["uniform8"] = _uniform_blocks(2, 1200, 51, 0.1)
     # few particles on a fine grid: cell boxes of ~100-500 points (the batched
-    # kernel's multi-word verdict masks) and beyond (point-by-point probes)
+    # kernel's multi-word verdict masks) and beyond (line windows)
     out["sparse8"] = _uniform_blocks(2, 150, 53, 0.2)
+    # fewer still: boxes of ~10^5 points with lines of 17-64 points (line
+    # windows) and one longer line (face-parallel probes) at 200^3
+    out["coarse8"] = _uniform_blocks(2, 40, 54, 0.3)
     pts, bid, counts, bounds = synthetic.plummer_blocks(2, 500, 4000, seed=52)
     out["plummer8"] = ingest.decomposed_blocks(pts, bid, bounds, 0.05)
     return out
@@ -52,7 +55,7 @@ def _bits(a):
 
 
 @pytest.mark.parametrize("name,g", [("uniform8", 33), ("uniform8", 40), ("plummer8", 33),
-                                    ("sparse8", 48), ("sparse8", 72)])
+                                    ("sparse8", 48), ("sparse8", 72), ("coarse8", 200)])
 def test_multiblock_reference_scan_bitexact(oracle, multiblocks, name, g):
     blocks = multiblocks[name]
     say("multiblock", name, g, [b.num_orig for b in blocks])

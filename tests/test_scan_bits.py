@@ -1,7 +1,8 @@
 """The cell scan's bit-scan line walk (tess2_amd/csrc/scan_bits.hpp) against
 the reference's point-by-point x walks (src/dense.cpp:1546-1620), on the
 host: every verdict pattern of lines up to 12 points with every start pair,
-and 2M random lines of 13-32 points.  The header is compiled with g++ with
+and 2M random lines of 13-64 points (with all-inside, all-outside and
+single-point lines of 64 at both ends).  The header is compiled with g++ with
 the HIP qualifiers defined away; the walk below is the reference's loop
 with PtInCell replaced by a bit lookup."""
 import os
@@ -76,13 +77,18 @@ int main()
     unsigned long long s = 88172645463325252ull;   // xorshift64
     for (int it = 0; it < 2000000; ++it) {
         s ^= s << 13; s ^= s >> 7; s ^= s << 17;
-        const int n0 = 13 + (int)(s % 20);
-        const unsigned long long all = (1ull << n0) - 1ull;
+        const int n0 = 13 + (int)(s % 52);
+        const unsigned long long all = n0 == 64 ? ~0ull : (1ull << n0) - 1ull;
         unsigned long long r = (s >> 11) & all;
         if (it & 1) r |= (s >> 3) & all;          // denser lines
         if (it % 3 == 0) r = all & ~(r & (r >> 1)); // long runs
         check(r, n0, (int)((s >> 40) % n0), (int)((s >> 50) % n0));
     }
+    const unsigned long long edge[] = {0ull, ~0ull, 1ull, 1ull << 63, 3ull << 62, 3ull,
+                                       ~1ull, ~(1ull << 63), 0x00ffff0000ffff00ull};
+    for (unsigned long long r : edge)
+        for (int xl = 0; xl < 64; ++xl)
+            for (int xr = 0; xr < 64; ++xr) check(r, 64, xl, xr);
     printf("cases %ld mismatches %ld\n", cases, bad);
     return bad != 0;
 }

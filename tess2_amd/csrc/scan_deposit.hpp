@@ -189,13 +189,33 @@ __device__ __forceinline__ void scan_deposit(
           : inside(p0 + (float)(xi) * gp.step[0], p1 + (float)(yi) * gp.step[1],             \
                    p2 + (float)(zi) * gp.step[2]))
 
+    // the points of runs[0, nb) (btot in all, run k's first at runs[k].w),
+    // 64 at a time: one lane per point, which finds its run in LDS
+    float mpp = 0.0f;
+    auto deposit_runs = [&](const int nb, const int btot) {
+        for (int base = 0; base < btot; base += 64) {
+            const int p = base + lane;
+            int r = 0;
+            for (int k = 1; k < nb; ++k)
+                if (p >= runs[k].w) r = k;
+            if (p < btot) {
+                const int4 rr = runs[r];
+                deposit_point(cmi0 + rr.z + (p - rr.w), cmi1 + rr.y, cmi2 + rr.x, mpp,
+                              cell_key(v, 0), gp, o, rt, self);
+            }
+        }
+    };
+
     // ---- 5. scanline search (src/dense.cpp:1486-1676) --------------------
     // Pass 0 counts the points and keeps the runs (one per z-y line) in
     // LDS.  A cell with more than RMAX runs scans again in pass 1 -- the
-    // state machine takes the same path -- and deposits each run as it is
-    // found, now that mass/N is known.
+    // state machine takes the same path -- now that mass/N is known, and
+    // deposits its runs as it finds them: RMAX at a time as the buffer fills
+    // in the LINE kernels, whose sparse inputs have many such cells, one at
+    // a time in the others (a cell adds at most one contribution to a grid
+    // point, so the order of its deposits does not matter).
     int nruns = 0, tot = 0;
-    float mpp = 0.0f;
+    int nb = 0, btot = 0;                 // pass 1: runs buffered, their points
     int lz = -1, ly0 = 0;                 // the line window: plane, first line
     unsigned long long lw = 0ull;         // and its verdicts
     const int lwin = 64 >> lb0;           // lines per window (lb0 <= 6 there)
@@ -287,7 +307,21 @@ __device__ __forceinline__ void scan_deposit(
                             runs[nruns] = make_int4(zi, yi, min_xi, tot);
                         nruns++;
                         tot += (max_xi - min_xi + 1);
+                    } else if (LINE) {
+                        if (nb == RMAX) {
+                            __syncthreads();
+                            deposit_runs(nb, btot);
+                            __syncthreads();
+                            nb = 0;
+                            btot = 0;
+                        }
+                        if (lane == 0) runs[nb] = make_int4(zi, yi, min_xi, btot);
+                        nb++;
+                        btot += max_xi - min_xi + 1;
                     } else {
+                        // one run at a time: the buffered deposit inside the
+                        // walk costs these kernels registers, and their inputs
+                        // rarely have cells of more than RMAX runs
                         const int len = max_xi - min_xi + 1;
                         for (int b0 = 0; b0 < len; b0 += 64)
                             if (b0 + lane < len)
@@ -319,23 +353,14 @@ __device__ __forceinline__ void scan_deposit(
 
     // ---- 6. deposit -------------------------------------------------------
     if (tot > 0 && nruns > RMAX) {
-        if (lane == 0) {   // deposited by pass 1
+        if (LINE) deposit_runs(nb, btot);   // pass 1's last runs
+        if (lane == 0) {
             o.status[v] = ST_SCAN;
             o.npts[v] = tot;
         }
     } else if (tot > 0) {
         mpp = gp.mass / (float)tot;   // src/dense.cpp:1692
-        for (int base = 0; base < tot; base += 64) {
-            const int p = base + lane;
-            int r = 0;
-            for (int k = 1; k < nruns; ++k)
-                if (p >= runs[k].w) r = k;
-            if (p < tot) {
-                const int4 rr = runs[r];
-                deposit_point(cmi0 + rr.z + (p - rr.w), cmi1 + rr.y, cmi2 + rr.x, mpp,
-                              cell_key(v, 0), gp, o, rt, self);
-            }
-        }
+        deposit_runs(nruns, tot);
         if (lane == 0) {
             o.status[v] = ST_SCAN;
             o.npts[v] = tot;

@@ -127,6 +127,9 @@ struct tess2_ctx {
     GridP *d_gps = nullptr;
     CellOut *d_outs = nullptr;
     Router *d_router = nullptr;
+    int *d_rmap = nullptr;              // the router's interval maps and table
+    int2 *d_rcand = nullptr;
+    size_t rmap_cap = 0, rcand_cap = 0;
     Chunk *d_chunks = nullptr;
     int nchunks = 0;
     unsigned *d_queue = nullptr;
@@ -187,7 +190,10 @@ static void drop_graph(tess2_ctx *c)
 static void free_multi(tess2_ctx *c)
 {
     void *ps[] = {c->d_ins, c->d_gps, c->d_outs, c->d_router, c->d_chunks, c->d_queue,
-                  c->d_stats, c->d_gpart, c->d_npts_tab, c->d_stex_tab};
+                  c->d_stats, c->d_gpart, c->d_npts_tab, c->d_stex_tab, c->d_rmap, c->d_rcand};
+    c->d_rmap = nullptr;
+    c->d_rcand = nullptr;
+    c->rmap_cap = c->rcand_cap = 0;
     c->d_npts_tab = nullptr;
     c->d_stex_tab = nullptr;
     c->homes_ok = false;
@@ -768,6 +774,74 @@ static BlockIn make_in(DevBlock &d)
     return in;
 }
 
+// The router's tables (Router::rmap / rcand) from the blocks' 3-D grid
+// ranges: a point that a cell deposits outside its own block goes to the
+// first block (staging order) other than its own whose range holds it
+// (src/dense.cpp:298-305).  The range ends cut each axis into intervals; a
+// box of intervals lies in a block's range entirely or not at all, so its
+// first two holders answer the question for every sender.  At most 127
+// blocks: at most 253 intervals per axis.
+static int build_route_table(tess2_ctx *c, Router &hr)
+{
+    const int nb = (int)c->blocks.size();
+    std::vector<int> cuts[3];
+    for (int a = 0; a < 3; ++a) {
+        for (int k = 0; k < nb; ++k) {
+            cuts[a].push_back(c->h_gps[k].bmin_idx[a]);
+            cuts[a].push_back(c->h_gps[k].bmin_idx[a] + c->h_gps[k].bnum_idx[a]);
+        }
+        std::sort(cuts[a].begin(), cuts[a].end());
+        cuts[a].erase(std::unique(cuts[a].begin(), cuts[a].end()), cuts[a].end());
+        hr.rbase[a] = cuts[a].front();
+        hr.rlen[a] = cuts[a].back() - cuts[a].front();
+        hr.rn[a] = (int)cuts[a].size() - 1;
+    }
+    hr.roff[0] = 0;
+    hr.roff[1] = hr.rlen[0];
+    hr.roff[2] = hr.rlen[0] + hr.rlen[1];
+    std::vector<int> map((size_t)hr.rlen[0] + hr.rlen[1] + hr.rlen[2]);
+    for (int a = 0; a < 3; ++a)
+        for (int i = 0; i < hr.rn[a]; ++i)
+            for (int g = cuts[a][i]; g < cuts[a][i + 1]; ++g)
+                map[(size_t)hr.roff[a] + (g - hr.rbase[a])] = i;
+    const size_t n1 = (size_t)hr.rn[1], n2 = (size_t)hr.rn[2];
+    std::vector<int2> cand((size_t)hr.rn[0] * n1 * n2, make_int2(-1, -1));
+    for (int k = 0; k < nb; ++k) {
+        int lo[3], hi[3];
+        for (int a = 0; a < 3; ++a) {
+            const int b0 = c->h_gps[k].bmin_idx[a], b1 = b0 + c->h_gps[k].bnum_idx[a];
+            lo[a] = (int)(std::lower_bound(cuts[a].begin(), cuts[a].end(), b0) - cuts[a].begin());
+            hi[a] = (int)(std::lower_bound(cuts[a].begin(), cuts[a].end(), b1) - cuts[a].begin());
+        }
+        for (int i0 = lo[0]; i0 < hi[0]; ++i0)
+            for (int i1 = lo[1]; i1 < hi[1]; ++i1)
+                for (int i2 = lo[2]; i2 < hi[2]; ++i2) {
+                    int2 &e = cand[((size_t)i0 * n1 + i1) * n2 + i2];
+                    if (e.x < 0) e.x = k;
+                    else if (e.y < 0) e.y = k;
+                }
+    }
+    int r = 0;
+    if (map.size() > c->rmap_cap) {
+        if ((r = dalloc(c, &c->d_rmap, map.size()))) return r;
+        c->rmap_cap = map.size();
+    }
+    if (cand.size() > c->rcand_cap) {
+        if ((r = dalloc(c, &c->d_rcand, cand.size()))) return r;
+        c->rcand_cap = cand.size();
+    }
+    // on the library stream, after any earlier run that still reads the
+    // tables; the host vectors must outlive the copies
+    HIPCHK(c, hipMemcpyAsync(c->d_rmap, map.data(), sizeof(int) * map.size(),
+                             hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->d_rcand, cand.data(), sizeof(int2) * cand.size(),
+                             hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    hr.rmap = c->d_rmap;
+    hr.rcand = c->d_rcand;
+    return 0;
+}
+
 // enqueue the whole path for every staged block
 // host-side set-up of a run: grid parameters, buffers, device descriptors
 static int prepare_run(tess2_ctx *c, const RunArgs &a)
@@ -842,10 +916,14 @@ static int prepare_run(tess2_ctx *c, const RunArgs &a)
         c->h_outs[k] = make_out(d, a.want_vol != 0);
     }
     if (c->desc_gen == c->gen && same_args(a, c->desc_args)) return 0;   // unchanged
-    Router hr;
+    Router hr{};
     hr.nblocks = nb;
     hr.gps = c->d_gps;
     hr.outs = c->d_outs;
+    if (nb > 1) {
+        int r = build_route_table(c, hr);
+        if (r) return r;
+    }
     HIPCHK(c, hipMemcpyAsync(c->d_ins, c->h_ins.data(), sizeof(BlockIn) * nb,
                              hipMemcpyHostToDevice, s));
     HIPCHK(c, hipMemcpyAsync(c->d_gps, c->h_gps.data(), sizeof(GridP) * nb,

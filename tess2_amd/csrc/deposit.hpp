@@ -85,10 +85,18 @@ __device__ __forceinline__ void deposit_point(int gx, int gy, int gz, float m, u
         return;
     }
     if (rt) {
-        for (int j = 0; j < rt->nblocks; ++j) {
-            if (j == self) continue;
-            const long long lj = block_lidx(gx, gy, gz, rt->gps[j], true);
-            if (lj >= 0) {
+        // the first block other than this one whose 3-D grid range holds
+        // the point: the first of the two candidates of its interval box
+        // that is not this block (two table loads, no walk over the blocks)
+        const int ix = gx - rt->rbase[0], iy = gy - rt->rbase[1], iz = gz - rt->rbase[2];
+        if (ix >= 0 && iy >= 0 && iz >= 0 && ix < rt->rlen[0] && iy < rt->rlen[1] &&
+            iz < rt->rlen[2]) {
+            const int a = rt->rmap[rt->roff[0] + ix], b = rt->rmap[rt->roff[1] + iy],
+                      c = rt->rmap[rt->roff[2] + iz];
+            const int2 cd = rt->rcand[((long long)a * rt->rn[1] + b) * rt->rn[2] + c];
+            const int j = cd.x != self ? cd.x : cd.y;
+            if (j >= 0) {
+                const long long lj = block_lidx(gx, gy, gz, rt->gps[j], true);
                 const unsigned key = 0x80000000u | ((unsigned)self << 24) | ckey;
                 push_point(lj, m, key, rt->gps[j].div, rt->outs[j]);
                 atomicAdd(&o.ctrs[C_ROUTED], 1u);

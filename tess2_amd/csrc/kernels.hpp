@@ -95,6 +95,14 @@ struct Router {
     int nblocks;
     const GridP *gps;
     const CellOut *outs;
+    // The ends of the blocks' 3-D grid ranges cut each axis into intervals:
+    // rmap[roff[a] + g - rbase[a]] is the interval of global index g on
+    // axis a (g in [rbase[a], rbase[a] + rlen[a])), rcand[(i0 * rn[1] + i1)
+    // * rn[2] + i2] the first two blocks (staging order, -1 for none) whose
+    // ranges hold the box of intervals (i0, i1, i2).
+    const int *rmap;
+    const int2 *rcand;
+    int rbase[3], rlen[3], roff[3], rn[3];
 };
 
 // one chunk of cells of one block for the batched launch

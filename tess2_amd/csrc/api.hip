@@ -132,6 +132,8 @@ struct tess2_ctx {
     size_t rmap_cap = 0, rcand_cap = 0;
     Chunk *d_chunks = nullptr;
     int nchunks = 0;
+    Chunk *d_items = nullptr;            // two consecutive cells of one block each
+    int nitems = 0;
     unsigned *d_queue = nullptr;
     XcdQueues xq{};
     std::vector<BlockIn> h_ins;
@@ -175,7 +177,10 @@ static bool same_args(const RunArgs &x, const RunArgs &y)
 // at 8 points per cell the masks cost 2 %).
 static int cell_kernel_for(const tess2_ctx *c, int ncells, unsigned long long ngrid)
 {
-    if (c->cell_kernel == 3) return 0;   // a multi-block choice; one block: one cell per wave
+    // the multi-block choices; one block: one cell per wave, or two with
+    // the multi-word masks
+    if (c->cell_kernel == 3) return 0;
+    if (c->cell_kernel == 4) return 2;
     if (c->cell_kernel >= 0) return c->cell_kernel;
     return ngrid >= 16ull * (unsigned long long)std::max(ncells, 1) ? 2 : 1;
 }
@@ -190,7 +195,10 @@ static void drop_graph(tess2_ctx *c)
 static void free_multi(tess2_ctx *c)
 {
     void *ps[] = {c->d_ins, c->d_gps, c->d_outs, c->d_router, c->d_chunks, c->d_queue,
-                  c->d_stats, c->d_gpart, c->d_npts_tab, c->d_stex_tab, c->d_rmap, c->d_rcand};
+                  c->d_stats, c->d_gpart, c->d_npts_tab, c->d_stex_tab, c->d_rmap, c->d_rcand,
+                  c->d_items};
+    c->d_items = nullptr;
+    c->nitems = 0;
     c->d_rmap = nullptr;
     c->d_rcand = nullptr;
     c->rmap_cap = c->rcand_cap = 0;
@@ -493,6 +501,16 @@ static int build_chunks(tess2_ctx *c)
     if (r) return r;
     if (!ch.empty())
         HIPCHK(c, hipMemcpy(c->d_chunks, ch.data(), sizeof(Chunk) * ch.size(),
+                            hipMemcpyHostToDevice));
+    // the item list of the queue-free launches: two cells at a time
+    std::vector<Chunk> it;
+    for (int b = 0; b < (int)c->blocks.size(); ++b)
+        for (int s0 = 0; s0 < c->blocks[b].norig; s0 += 2)
+            it.push_back(Chunk{b, s0, std::min(2, c->blocks[b].norig - s0), 0});
+    c->nitems = (int)it.size();
+    if ((r = dalloc(c, &c->d_items, std::max<size_t>(it.size(), 1)))) return r;
+    if (!it.empty())
+        HIPCHK(c, hipMemcpy(c->d_items, it.data(), sizeof(Chunk) * it.size(),
                             hipMemcpyHostToDevice));
     return 0;
 }
@@ -1026,7 +1044,11 @@ static int enqueue_kernels(tess2_ctx *c, const RunArgs &a)
         // two-cell kernel measured 24.1 vs 22.7 ms (its 48-tet capacity
         // defers 6,241 cells of the dense cores) and the early loads 23.6 vs
         // 24.0 ms
-        if (c->cell_kernel <= 0 || c->cell_kernel == 3) {
+        if (c->cell_kernel == 4) {
+            launch_cells_mc_items(c->d_ins, c->d_gps, c->d_outs, c->d_router, c->d_items,
+                                  c->nitems, 1, s);
+            launch_cells_ref_slow_batched(c->d_ins, c->d_router, 1, s);
+        } else if (c->cell_kernel <= 0 || c->cell_kernel == 3) {
             launch_cells_ref_batched(c->d_ins, c->d_gps, c->d_outs, c->d_router, c->d_chunks,
                                      c->xq, c->d_queue, c->ncu * c->waves_per_cu, 1, s,
                                      c->cell_kernel != 0);
@@ -1231,7 +1253,7 @@ extern "C" int tess2_set_option(tess2_ctx *c, int option, int value)
         c->waves_per_cu = value;
         return 0;
     case TESS2_OPT_CELL_KERNEL:
-        if (value < -1 || value > 3) return fail(c, TESS2_E_ARG, "cell kernel: -1..3");
+        if (value < -1 || value > 4) return fail(c, TESS2_E_ARG, "cell kernel: -1..4");
         if (value != c->cell_kernel) c->gen++;   // a recorded graph is recorded again
         c->cell_kernel = value;
         return 0;
@@ -1314,6 +1336,15 @@ extern "C" int tess2_autotune(tess2_ctx *c, int reps, float *best_ms)
             (r = tess2_set_option(c, TESS2_OPT_CHUNK_LARGE, bc[2])) ||
             (r = tess2_set_option(c, TESS2_OPT_WAVES_PER_CU, bw)))
             return r;
+        // the queue-free two-cell launch, unless a cell kernel was asked for
+        if (c->cell_kernel == -1) {
+            float t = 0.0f;
+            if ((r = tess2_set_option(c, TESS2_OPT_CELL_KERNEL, 4)) ||
+                (r = time_runs(c, 1, &t)) || (r = time_runs(c, reps, &t)))
+                return r;
+            if (t < best) best = t;
+            else if ((r = tess2_set_option(c, TESS2_OPT_CELL_KERNEL, -1))) return r;
+        }
         float t = 0.0f;
         if ((r = time_runs(c, 1, &t))) return r;   // leave the results of the chosen split
     }

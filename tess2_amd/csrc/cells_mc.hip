@@ -500,6 +500,45 @@ __global__ __launch_bounds__(64, MINW) void k_cells_mc_batched(
     }
 }
 
+// All blocks of a run in one launch without a work queue: workgroup b runs
+// item xcd_item(b) of the item list (CPW consecutive cells of one block, the
+// blocks in staging order: each XCD gets one contiguous part of the list),
+// with the block's descriptors read per workgroup.  The dispatcher balances
+// the waves within an XCD, not across them: this beats the persistent
+// kernels where the blocks cost alike (8 uniform blocks: 5.6 vs 7.3 ms) and
+// loses where they do not (64 Plummer blocks: 24.1 vs 19.7 ms); dealing the
+// items to the XCDs in tiles instead loses L2 locality (35 ms there).
+template <int CPW, int SG, int HG, int FG, int RMAX, int MINW, bool WIDE>
+__global__ __launch_bounds__(64, MINW) void k_cells_mc_items(
+    const BlockIn *__restrict__ ins, const GridP *__restrict__ gps,
+    const CellOut *__restrict__ outs, const Router *__restrict__ rt,
+    const Chunk *__restrict__ items, int nitems, int want_density)
+{
+    __shared__ __align__(16) unsigned char smem[CPW * McLayout<SG, HG>::BYTES];
+    const int item = xcd_item((int)blockIdx.x, nitems);
+    if (item < 0) return;
+    const Chunk ch = items[item];
+    const BlockIn in = ins[ch.block];
+    const GridP gp = gps[ch.block];
+    const CellOut o = outs[ch.block];
+    const int g = (int)(threadIdx.x & 63) / (64 / CPW);
+    const int kk = ch.start + g;
+    const int v = g < ch.count ? (in.order ? in.order[kk] : kk) : -1;
+    cells_mc_body<CPW, SG, HG, FG, RMAX, WIDE, true>(v, in, gp, o, want_density, rt, ch.block,
+                                                     smem);
+}
+
+void launch_cells_mc_items(const BlockIn *ins, const GridP *gps, const CellOut *outs,
+                           const Router *rt, const Chunk *items, int nitems, int want_density,
+                           hipStream_t s)
+{
+    if (nitems <= 0) return;
+    // 6 waves per SIMD (80 VGPRs, 2 spilled): 5.6 ms on 8 uniform blocks
+    // against 6.4 at 5 waves (no spills) and 6.3 at 4
+    hipLaunchKernelGGL((k_cells_mc_items<2, 48, 64, 48, 48, 6, true>), dim3(xcd_grid(nitems)),
+                       dim3(64), 0, s, ins, gps, outs, rt, items, nitems, want_density);
+}
+
 template <int CPW, int SG, int HG, int FG, int RMAX, int MINW = 1, bool WIDE = false>
 static void launch_mc(const BlockIn &in, const GridP &gp, const CellOut &o, int want_density,
                       hipStream_t s)

@@ -206,6 +206,11 @@ void launch_cells_ref_slow_batched(const BlockIn *ins, const Router *rt, int wan
                                    hipStream_t s);
 // the two-cell kernel over all blocks in one persistent launch (the deferred
 // cells then need launch_cells_ref_batched's large-capacity pass)
+// the batched launch over an item list (two consecutive cells of one block
+// per item), one workgroup per item, no work queue
+void launch_cells_mc_items(const BlockIn *ins, const GridP *gps, const CellOut *outs,
+                           const Router *rt, const Chunk *items, int nitems, int want_density,
+                           hipStream_t s);
 void launch_cells_mc_batched(const BlockIn *ins, const GridP *gps, const CellOut *outs,
                              const Router *rt, const Chunk *chunks, const XcdQueues &xq,
                              unsigned *queue, int waves, int want_density, hipStream_t s);

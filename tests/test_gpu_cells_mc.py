@@ -90,14 +90,15 @@ def _batched_blocks(name):
     return _BATCHED[name]
 
 
-@pytest.mark.parametrize("variant", [0, 1, 3])
+@pytest.mark.parametrize("variant", [0, 1, 3, 4])
 @pytest.mark.parametrize("name,g", [("uniform8", 48), ("sparse8", 48), ("sparse8", 72),
                                     ("plummer8", 48), ("coarse8", 200)])
 def test_cells_mc_batched_bitexact(oracle, name, g, variant):
-    """The persistent kernels over several blocks (0: one cell per wave; 1:
+    """The multi-block cell kernels (persistent, 0: one cell per wave; 1:
     two cells per wave; 3, the default: one cell per wave with the star
-    search's record loads issued before the table work), against the
-    oracle's multi-block model: per-block statuses, counts, densities.  The
+    search's record loads issued before the table work; 4: two cells per
+    wave, one workgroup per two cells, no work queue), against the oracle's
+    multi-block model: per-block statuses, counts, densities.  The
     sparse cases put cell boxes past the 512-point masks onto the line
     windows: lines of up to 16 points at 72^3, of 17-64 points (and one
     longer, on face-parallel probes) in coarse8 at 200^3."""

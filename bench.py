@@ -308,6 +308,9 @@ def main():
     ap.add_argument("--calibrate", action="store_true",
                     help="run a plain copy kernel (256 MiB read, 256 MiB written) three times "
                          "and exit: the reference the HBM traffic counters are scaled by")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="several blocks: keep the default work split (no autotune), "
+                         "for counter runs")
     ap.add_argument("--no-graph", action="store_true",
                     help="time eager launches instead of graph replays (for tracing tools)")
     ap.add_argument("--save-block", default=None,
@@ -364,7 +367,7 @@ def main():
     graph = 0 if args.no_graph else 1
     run.set_option("graph", graph)
     tune = None
-    if multi:
+    if multi and not args.no_tune:
         best, opts = run.autotune(reps=3)
         tune = {"best_ms": best, **opts}
         log(f"[autotune] {best:.3f} ms with {opts}")
@@ -401,10 +404,11 @@ def main():
     tf = os.path.join(ROOT, "profiles", f"traffic_{c['name']}.json")
     if os.path.exists(tf):
         traffic = json.load(open(tf)).get("hbm_bytes_per_launch")
+    ck = run.get_option("cell_kernel")
     if multi:
-        kernel = "k_cells_ref_batched"
+        kernel = {1: "k_cells_mc_batched", 2: "k_cells_mc_batched",
+                  4: "k_cells_mc_items"}.get(ck, "k_cells_ref_batched")
     else:
-        ck = run.get_option("cell_kernel")
         kernel = "k_cells_ref" if ck == 0 else "k_cells_mc"
     cfg = {"workload": c["name"], "blocks": len(blocks), "particles": N, "ghosts": N_all - N,
            "tets": T, "grid": list(grid), "grid_points": G, "assignment": "reference_scan",

@@ -491,7 +491,7 @@ __global__ __launch_bounds__(64, MINW) void k_cells_mc_batched(
         const GridP gp = gps[ch.block];
         const CellOut o = outs[ch.block];
         for (int j = 0; j < ch.count; j += CPW) {
-            const int kk = ch.start + j + g;
+            const int kk = ch.start + (j + g) * ch.stride;
             const int v = j + g < ch.count ? (in.order ? in.order[kk] : kk) : -1;
             cells_mc_body<CPW, SG, HG, FG, RMAX, WIDE, true>(v, in, gp, o, want_density, rt,
                                                              ch.block, smem);
@@ -512,31 +512,41 @@ template <int CPW, int SG, int HG, int FG, int RMAX, int MINW, bool WIDE>
 __global__ __launch_bounds__(64, MINW) void k_cells_mc_items(
     const BlockIn *__restrict__ ins, const GridP *__restrict__ gps,
     const CellOut *__restrict__ outs, const Router *__restrict__ rt,
-    const Chunk *__restrict__ items, int nitems, int want_density)
+    const Chunk *__restrict__ items, XcdQueues xr, int want_density)
 {
     __shared__ __align__(16) unsigned char smem[CPW * McLayout<SG, HG>::BYTES];
-    const int item = xcd_item((int)blockIdx.x, nitems);
-    if (item < 0) return;
+    const int x = (int)(blockIdx.x & 7);
+    int s0 = xr.start[0], s1 = xr.start[1];
+#pragma unroll
+    for (int i = 1; i < 8; ++i)
+        if (x == i) {
+            s0 = xr.start[i];
+            s1 = xr.start[i + 1];
+        }
+    const int item = s0 + (int)(blockIdx.x >> 3);
+    if (item >= s1) return;
     const Chunk ch = items[item];
     const BlockIn in = ins[ch.block];
     const GridP gp = gps[ch.block];
     const CellOut o = outs[ch.block];
     const int g = (int)(threadIdx.x & 63) / (64 / CPW);
-    const int kk = ch.start + g;
+    const int kk = ch.start + g * ch.stride;
     const int v = g < ch.count ? (in.order ? in.order[kk] : kk) : -1;
     cells_mc_body<CPW, SG, HG, FG, RMAX, WIDE, true>(v, in, gp, o, want_density, rt, ch.block,
                                                      smem);
 }
 
 void launch_cells_mc_items(const BlockIn *ins, const GridP *gps, const CellOut *outs,
-                           const Router *rt, const Chunk *items, int nitems, int want_density,
-                           hipStream_t s)
+                           const Router *rt, const Chunk *items, const XcdQueues &xr,
+                           int want_density, hipStream_t s)
 {
-    if (nitems <= 0) return;
+    int most = 0;
+    for (int x = 0; x < 8; ++x) most = std::max(most, xr.start[x + 1] - xr.start[x]);
+    if (most <= 0) return;
     // 6 waves per SIMD (80 VGPRs, 2 spilled): 5.6 ms on 8 uniform blocks
     // against 6.4 at 5 waves (no spills) and 6.3 at 4
-    hipLaunchKernelGGL((k_cells_mc_items<2, 48, 64, 48, 48, 6, true>), dim3(xcd_grid(nitems)),
-                       dim3(64), 0, s, ins, gps, outs, rt, items, nitems, want_density);
+    hipLaunchKernelGGL((k_cells_mc_items<2, 48, 64, 48, 48, 6, true>), dim3(8 * most),
+                       dim3(64), 0, s, ins, gps, outs, rt, items, xr, want_density);
 }
 
 template <int CPW, int SG, int HG, int FG, int RMAX, int MINW = 1, bool WIDE = false>

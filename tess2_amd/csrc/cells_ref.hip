@@ -494,7 +494,7 @@ __global__ __launch_bounds__(64, 6) void k_cells_ref_batched(const BlockIn *__re
         const GridP gp = gps[ch.block];
         const CellOut o = outs[ch.block];
         for (int j = 0; j < ch.count; ++j) {
-            const int k = ch.start + j;
+            const int k = ch.start + j * ch.stride;
             const int v = in.order ? in.order[k] : k;
             cell_ref_body<NS, NF, RMAX, true, SPEC>(v, in, gp, o, want_density, true, s_v, s_ns,
                                                     s_cc, s_u,
@@ -514,14 +514,32 @@ __global__ __launch_bounds__(64) void k_cells_ref_slow_batched(const BlockIn *__
     __shared__ float4 s_cc[64 * NS];
     __shared__ int4 s_u[(64 * NS > RMAX) ? 64 * NS : RMAX];
     __shared__ int s_h[hash_ints<NS>()];
+    // the lists of up to 64 blocks at a time, concatenated and dealt to the
+    // workgroups in turn (one block per lane: list lengths and their
+    // inclusive prefix sum), so that no workgroup takes one cell of every
+    // block while others idle
+    const int lane = lane_id();
     const int nb = rt->nblocks;
-    for (int b = 0; b < nb; ++b) {
-        const CellOut &o = rt->outs[b];
-        unsigned n = o.ctrs[C_SLOW];
-        if (n > o.slow_cap) n = o.slow_cap;
-        for (unsigned k = blockIdx.x; k < n; k += gridDim.x) {
-            cell_ref_body<NS, NF, RMAX, true>(o.slow[k], ins[b], rt->gps[b], o, want_density,
-                                              false, s_v, s_ns, s_cc, s_u, s_h, rt, b);
+    for (int b0 = 0; b0 < nb; b0 += 64) {
+        unsigned n = 0;
+        if (b0 + lane < nb) {
+            const CellOut &o = rt->outs[b0 + lane];
+            n = min(o.ctrs[C_SLOW], o.slow_cap);
+        }
+        unsigned incl = n;
+        for (int d = 1; d < 64; d <<= 1) {
+            const unsigned t = __shfl_up(incl, d);
+            if (lane >= d) incl += t;
+        }
+        const unsigned total = __shfl(incl, 63);
+        for (unsigned g = blockIdx.x; g < total; g += gridDim.x) {
+            const int lb = __ffsll((unsigned long long)__ballot(incl > g)) - 1;
+            const unsigned first = __shfl(incl - n, lb);
+            const int b = b0 + lb;
+            const CellOut &o = rt->outs[b];
+            cell_ref_body<NS, NF, RMAX, true>(o.slow[g - first], ins[b], rt->gps[b], o,
+                                              want_density, false, s_v, s_ns, s_cc, s_u, s_h,
+                                              rt, b);
             __syncthreads();
         }
     }

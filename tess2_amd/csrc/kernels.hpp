@@ -108,9 +108,9 @@ struct Router {
 // one chunk of cells of one block for the batched launch
 struct Chunk {
     int block;
-    int start;   // position in the block's cell order
+    int start;    // position in the block's cell order of the first cell
     int count;
-    int pad;
+    int stride;   // cells start, start + stride, ... (count of them)
 };
 
 // The chunk list split into 8 contiguous ranges, one per XCD (workgroup b
@@ -209,8 +209,8 @@ void launch_cells_ref_slow_batched(const BlockIn *ins, const Router *rt, int wan
 // the batched launch over an item list (two consecutive cells of one block
 // per item), one workgroup per item, no work queue
 void launch_cells_mc_items(const BlockIn *ins, const GridP *gps, const CellOut *outs,
-                           const Router *rt, const Chunk *items, int nitems, int want_density,
-                           hipStream_t s);
+                           const Router *rt, const Chunk *items, const XcdQueues &xr,
+                           int want_density, hipStream_t s);
 void launch_cells_mc_batched(const BlockIn *ins, const GridP *gps, const CellOut *outs,
                              const Router *rt, const Chunk *chunks, const XcdQueues &xq,
                              unsigned *queue, int waves, int want_density, hipStream_t s);

@@ -142,7 +142,7 @@ struct tess2_ctx {
     std::vector<CellOut> h_outs;
     // batched schedule (tess2_autotune): chunk size per block size class and
     // persistent waves per CU
-    int chunk_small = 16, chunk_medium = 32, chunk_large = 64, waves_per_cu = 28;
+    int chunk_small = 16, chunk_medium = 32, chunk_large = 64, waves_per_cu = 24;
     // single-block cell kernel: -1 = chosen per run (cell_kernel_for), 0 =
     // one cell per wave, 1 / 2 = two cells per wave in the geometry phase
     // (launch_cells_mc), 2 with the multi-word verdict masks

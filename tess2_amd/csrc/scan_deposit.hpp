@@ -110,8 +110,9 @@ __device__ __forceinline__ void scan_deposit(
     // ~2 n0 face-parallel probes of the two x walks.
     const bool use_line = LINE && !use_mask && !use_wide && n0 <= 64;
     unsigned long long mk = 0ull;
-    unsigned long long w1 = 0ull, w2 = 0ull, w3 = 0ull, w4 = 0ull, w5 = 0ull, w6 = 0ull,
-                       w7 = 0ull;
+    // the wide masks' words: lane c holds word c (low / high half), read at
+    // a wave-uniform lane index (no select chain over eight scalar words)
+    unsigned wlo = 0u, whi = 0u;
     if ((use_mask || use_wide || use_line) && !faces_staged) {
         __syncthreads();
 #pragma unroll
@@ -142,14 +143,10 @@ __device__ __forceinline__ void scan_deposit(
             const float qz = p2 + (float)z * gp.step[2];
             const unsigned long long m =
                 __ballot(p < nbox && lds_verdict(qx, qy, qz, F, s_fa, s_fb, eps));
-            if (c == 0) mk = m;
-            else if (c == 1) w1 = m;
-            else if (c == 2) w2 = m;
-            else if (c == 3) w3 = m;
-            else if (c == 4) w4 = m;
-            else if (c == 5) w5 = m;
-            else if (c == 6) w6 = m;
-            else w7 = m;
+            if (lane == c) {
+                wlo = (unsigned)m;
+                whi = (unsigned)(m >> 32);
+            }
         }
     }
     if (use_mask) {
@@ -180,8 +177,8 @@ __device__ __forceinline__ void scan_deposit(
     };
 
 #define WORD(k)                                                                               \
-((k) == 0 ? mk : (k) == 1 ? w1 : (k) == 2 ? w2 : (k) == 3 ? w3 : (k) == 4 ? w4 : (k) == 5 ? w5 \
- : (k) == 6 ? w6 : w7)
+(((unsigned long long)(unsigned)__builtin_amdgcn_readlane((int)whi, (k)) << 32) |             \
+ (unsigned)__builtin_amdgcn_readlane((int)wlo, (k)))
 #define PROBE(xi, yi, zi)                                                                     \
 (use_mask ? (bool)((mk >> ((xi) | ((yi) << lb0) | ((zi) << lb01))) & 1ull)                 \
  : use_wide ? wide_bit(WORD(((xi) + n0 * ((yi) + n1 * (zi))) >> 6),                        \

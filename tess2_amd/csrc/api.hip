@@ -940,10 +940,10 @@ static int prepare_run(tess2_ctx *c, const RunArgs &a)
     if (c->desc_gen == c->gen && same_args(a, c->desc_args)) return 0;   // unchanged
     if (nb > 1) {
         // XCD ranges of the item list by estimated cost: a cell, plus its
-        // share of the block's grid points at 0.022 cells each (measured on
-        // 64 Plummer blocks: ~68k clocks per cell, ~1.5k per deposited
-        // point; ranges by cell count there: 24.1 ms, by this cost: 21.3)
-        const double beta = 0.022;
+        // share of the block's grid points at 0.006 cells each (a least-
+        // squares fit of the per-block item times of this launch on 64
+        // Plummer blocks; ranges by cell count there: 24.1 ms, by cost 21.3)
+        const double beta = 0.006;
         std::vector<double> cum(c->nitems + 1, 0.0);
         int it = 0;
         for (int k = 0; k < nb; ++k) {

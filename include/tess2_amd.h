@@ -281,8 +281,8 @@ enum {
                                    * one block runs 0); 4 several blocks in one launch
                                    * without the work queue, a workgroup per two cells,
                                    * two cells per wave as 2 (one block runs 2);
-                                   * tess2_autotune picks 4 over the default when it is
-                                   * faster and the option is -1 */
+                                   * with the option at -1, tess2_autotune also times
+                                   * 1 and 4 and keeps the fastest */
 };
 
 /* Set / read one option.  Chunk sizes 1..4096, waves per CU 1..64, cell
@@ -292,7 +292,7 @@ int tess2_get_option(tess2_ctx *ctx, int option, int *value);
 
 /* Time the last tess2_run's arguments under each candidate work split of the
  * multi-block cell kernel (chunk sizes x persistent waves per CU), and, when
- * TESS2_OPT_CELL_KERNEL is -1, the queue-free launch (cell kernel 4), `reps`
+ * TESS2_OPT_CELL_KERNEL is -1, the two-cell kernels (cell kernels 1 and 4), `reps`
  * runs each, keep the fastest, and leave it set.  Results of the staged
  * blocks are those of the last timed run (identical for every candidate).
  * best_ms (may be NULL) receives the fastest median run time. */

@@ -1018,7 +1018,7 @@ static int enqueue_kernels(tess2_ctx *c, const RunArgs &a)
                 const int ck = nb > 1 ? 0 : cell_kernel_for(c, d.norig, d.ngrid);
                 if (ck == 0)
                     launch_cells_ref(in, gp, ov, 0, s);
-                else if (launch_cells_mc(in, gp, ov, 0, ck, s))
+                else if (launch_cells_mc(in, c->d_gps + k, ov, 0, ck, s))
                     return fail(c, TESS2_E_ARG, "unknown cell kernel %d", ck);
                 launch_cells_ref_slow(in, gp, ov, 0, s);
             }
@@ -1056,7 +1056,7 @@ static int enqueue_kernels(tess2_ctx *c, const RunArgs &a)
         const int ck = cell_kernel_for(c, c->blocks[0].norig, c->blocks[0].ngrid);
         if (ck == 0)
             launch_cells_ref(c->h_ins[0], c->h_gps[0], c->h_outs[0], 1, s);
-        else if (launch_cells_mc(c->h_ins[0], c->h_gps[0], c->h_outs[0], 1, ck, s))
+        else if (launch_cells_mc(c->h_ins[0], c->d_gps, c->h_outs[0], 1, ck, s))
             return fail(c, TESS2_E_ARG, "unknown cell kernel %d", ck);
         if (ev) HIPCHK(c, hipEventRecord(c->ev_k1, s));
         launch_cells_ref_slow(c->h_ins[0], c->h_gps[0], c->h_outs[0], 1, s);
@@ -1362,14 +1362,21 @@ extern "C" int tess2_autotune(tess2_ctx *c, int reps, float *best_ms)
             (r = tess2_set_option(c, TESS2_OPT_CHUNK_LARGE, bc[2])) ||
             (r = tess2_set_option(c, TESS2_OPT_WAVES_PER_CU, bw)))
             return r;
-        // the queue-free two-cell launch, unless a cell kernel was asked for
+        // the persistent and the queue-free two-cell kernels with the chosen
+        // split, unless a cell kernel was asked for
         if (c->cell_kernel == -1) {
-            float t = 0.0f;
-            if ((r = tess2_set_option(c, TESS2_OPT_CELL_KERNEL, 4)) ||
-                (r = time_runs(c, 1, &t)) || (r = time_runs(c, reps, &t)))
-                return r;
-            if (t < best) best = t;
-            else if ((r = tess2_set_option(c, TESS2_OPT_CELL_KERNEL, -1))) return r;
+            int bk = -1;
+            for (int k : {1, 4}) {
+                float t = 0.0f;
+                if ((r = tess2_set_option(c, TESS2_OPT_CELL_KERNEL, k)) ||
+                    (r = time_runs(c, 1, &t)) || (r = time_runs(c, reps, &t)))
+                    return r;
+                if (t < best) {
+                    best = t;
+                    bk = k;
+                }
+            }
+            if ((r = tess2_set_option(c, TESS2_OPT_CELL_KERNEL, bk))) return r;
         }
         float t = 0.0f;
         if ((r = time_runs(c, 1, &t))) return r;   // leave the results of the chosen split
@@ -1480,7 +1487,7 @@ extern "C" int tess2_volumes(tess2_ctx *c, const tess2_block_t *block, float *ou
     GridP gp{};
     const int ck = c->cell_kernel >= 0 ? c->cell_kernel : 1;
     if (ck == 0) launch_cells_ref(in, gp, o, 0, s);
-    else launch_cells_mc(in, gp, o, 0, ck, s);
+    else launch_cells_mc(in, nullptr, o, 0, ck, s);   // no density: the grid is not read
     launch_cells_ref_slow(in, gp, o, 0, s);
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipMemcpyAsync(d.hm->ctrs, d.ctrs, sizeof(unsigned) * C_NCTR, hipMemcpyDeviceToHost,

@@ -86,7 +86,7 @@ struct McLayout {
 // group regions of McLayout<SG, HG>::BYTES.  rt / self route off-block
 // points to the other blocks of a multi-block run.
 template <int CPW, int SG, int HG, int FG, int RMAX, bool WIDE, bool LINE>
-__device__ __forceinline__ void cells_mc_body(const int v, const BlockIn &in, const GridP &gp,
+__device__ __forceinline__ void cells_mc_body(const int v, const BlockIn &in, const GridP *gpp,
                                               const CellOut &o, const int want_density,
                                               const Router *rt, const int self,
                                               unsigned char *smem)
@@ -443,7 +443,7 @@ __device__ __forceinline__ void cells_mc_body(const int v, const BlockIn &in, co
         const float4 siteq = in.pts[vq];
         scan_deposit<1, RMAX, WIDE, LINE>(vq, siteq, Fq, nx1, ny1, nz1, fx1, fy1, fz1,
                                      rdl_f(mnx, ql), rdl_f(mny, ql), rdl_f(mnz, ql),
-                                     rdl_f(mxx, ql), rdl_f(mxy, ql), rdl_f(mxz, ql), gp, o,
+                                     rdl_f(mxx, ql), rdl_f(mxy, ql), rdl_f(mxz, ql), *gpp, o,
                                      rt, self, const_cast<float4 *>(faq),
                                      const_cast<float2 *>(fbq), true, runs);
         __syncthreads();
@@ -451,7 +451,8 @@ __device__ __forceinline__ void cells_mc_body(const int v, const BlockIn &in, co
 }
 
 template <int CPW, int SG, int HG, int FG, int RMAX, int MINW, bool WIDE>
-__global__ __launch_bounds__(64, MINW) void k_cells_mc(BlockIn in, GridP gp, CellOut o,
+__global__ __launch_bounds__(64, MINW) void k_cells_mc(BlockIn in,
+                                                       const GridP *__restrict__ gp, CellOut o,
                                                        int want_density)
 {
     __shared__ __align__(16) unsigned char smem[CPW * McLayout<SG, HG>::BYTES];
@@ -488,13 +489,12 @@ __global__ __launch_bounds__(64, MINW) void k_cells_mc_batched(
         }
         const Chunk ch = chunks[c];
         const BlockIn in = ins[ch.block];
-        const GridP gp = gps[ch.block];
         const CellOut o = outs[ch.block];
         for (int j = 0; j < ch.count; j += CPW) {
             const int kk = ch.start + (j + g) * ch.stride;
             const int v = j + g < ch.count ? (in.order ? in.order[kk] : kk) : -1;
-            cells_mc_body<CPW, SG, HG, FG, RMAX, WIDE, true>(v, in, gp, o, want_density, rt,
-                                                             ch.block, smem);
+            cells_mc_body<CPW, SG, HG, FG, RMAX, WIDE, true>(v, in, gps + ch.block, o,
+                                                             want_density, rt, ch.block, smem);
             __syncthreads();
         }
     }
@@ -527,13 +527,12 @@ __global__ __launch_bounds__(64, MINW) void k_cells_mc_items(
     if (item >= s1) return;
     const Chunk ch = items[item];
     const BlockIn in = ins[ch.block];
-    const GridP gp = gps[ch.block];
     const CellOut o = outs[ch.block];
     const int g = (int)(threadIdx.x & 63) / (64 / CPW);
     const int kk = ch.start + g * ch.stride;
     const int v = g < ch.count ? (in.order ? in.order[kk] : kk) : -1;
-    cells_mc_body<CPW, SG, HG, FG, RMAX, WIDE, true>(v, in, gp, o, want_density, rt, ch.block,
-                                                     smem);
+    cells_mc_body<CPW, SG, HG, FG, RMAX, WIDE, true>(v, in, gps + ch.block, o, want_density, rt,
+                                                     ch.block, smem);
 }
 
 void launch_cells_mc_items(const BlockIn *ins, const GridP *gps, const CellOut *outs,
@@ -550,7 +549,7 @@ void launch_cells_mc_items(const BlockIn *ins, const GridP *gps, const CellOut *
 }
 
 template <int CPW, int SG, int HG, int FG, int RMAX, int MINW = 1, bool WIDE = false>
-static void launch_mc(const BlockIn &in, const GridP &gp, const CellOut &o, int want_density,
+static void launch_mc(const BlockIn &in, const GridP *gp, const CellOut &o, int want_density,
                       hipStream_t s)
 {
     const int nitems = (in.ncells + CPW - 1) / CPW;
@@ -568,7 +567,7 @@ void launch_cells_mc_batched(const BlockIn *ins, const GridP *gps, const CellOut
                            0, s, ins, gps, outs, rt, chunks, xq, queue, want_density);
 }
 
-int launch_cells_mc(const BlockIn &in, const GridP &gp, const CellOut &o, int want_density,
+int launch_cells_mc(const BlockIn &in, const GridP *gp, const CellOut &o, int want_density,
                     int variant, hipStream_t s)
 {
     if (in.ncells <= 0) return 0;

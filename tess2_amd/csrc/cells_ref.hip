@@ -80,7 +80,7 @@ template <int NS> constexpr int hash_entries() { return 2 * 64 * NS; }
 template <int NS> constexpr int hash_ints() { return 2 * hash_entries<NS>() + 2 * 64 * NS; }
 
 template <int NS, int NF, int RMAX, bool WIDE = false, bool SPEC = false>
-__device__ void cell_ref_body(const int v, const BlockIn &in, const GridP &gp,
+__device__ void cell_ref_body(const int v, const BlockIn &in, const GridP *gpp,
                               const CellOut &o, const int want_density, const bool can_defer,
                               int4 *s_v, int *s_ns, float4 *s_cc, int4 *s_u, int *s_h,
                               const Router *rt = nullptr, const int self = 0)
@@ -403,7 +403,7 @@ __device__ void cell_ref_body(const int v, const BlockIn &in, const GridP &gp,
             if (isnan(c00.z)) { mnz = c00.z; mxz = c00.z; }
         }
         scan_deposit<NF, RMAX, WIDE, WIDE>(v, site, F, fnx, fny, fnz, ffx, ffy, ffz, mnx, mny, mnz,
-                                     mxx, mxy, mxz, gp, o, rt, self, s_cc,
+                                     mxx, mxy, mxz, *gpp, o, rt, self, s_cc,
                                      reinterpret_cast<float2 *>(s_v), false, s_u);
         return;
     }
@@ -434,7 +434,7 @@ __global__ __launch_bounds__(64) void k_cells_ref(BlockIn in, GridP gp, CellOut 
     const int k = xcd_item((int)blockIdx.x, in.ncells);
     if (k < 0) return;
     const int v = in.order ? in.order[k] : k;
-    cell_ref_body<NS, NF, RMAX>(v, in, gp, o, want_density, true, s_v, s_ns, s_cc, s_u, s_h);
+    cell_ref_body<NS, NF, RMAX>(v, in, &gp, o, want_density, true, s_v, s_ns, s_cc, s_u, s_h);
 }
 
 // large-capacity pass over the cells the fast kernel deferred; the grid is
@@ -452,7 +452,8 @@ __global__ __launch_bounds__(64) void k_cells_ref_slow(BlockIn in, GridP gp, Cel
     if (n > o.slow_cap) n = o.slow_cap;
     for (unsigned k = blockIdx.x; k < n; k += gridDim.x) {
         const int v = o.slow[k];
-        cell_ref_body<NS, NF, RMAX>(v, in, gp, o, want_density, false, s_v, s_ns, s_cc, s_u, s_h);
+        cell_ref_body<NS, NF, RMAX>(v, in, &gp, o, want_density, false, s_v, s_ns, s_cc, s_u,
+                                    s_h);
         __syncthreads();
     }
 }
@@ -491,12 +492,14 @@ __global__ __launch_bounds__(64, 6) void k_cells_ref_batched(const BlockIn *__re
         const Chunk ch = chunks[c];
         // read-only descriptors (scalar loads; rt->gps/outs alias them for routing)
         const BlockIn in = ins[ch.block];
+        // a register copy: read from memory inside the cell body this
+        // kernel measured 6 % slower (the two-cell kernels gain from it)
         const GridP gp = gps[ch.block];
         const CellOut o = outs[ch.block];
         for (int j = 0; j < ch.count; ++j) {
             const int k = ch.start + j * ch.stride;
             const int v = in.order ? in.order[k] : k;
-            cell_ref_body<NS, NF, RMAX, true, SPEC>(v, in, gp, o, want_density, true, s_v, s_ns,
+            cell_ref_body<NS, NF, RMAX, true, SPEC>(v, in, &gp, o, want_density, true, s_v, s_ns,
                                                     s_cc, s_u,
                                         s_h, rt, ch.block);
             __syncthreads();
@@ -537,7 +540,7 @@ __global__ __launch_bounds__(64) void k_cells_ref_slow_batched(const BlockIn *__
             const unsigned first = __shfl(incl - n, lb);
             const int b = b0 + lb;
             const CellOut &o = rt->outs[b];
-            cell_ref_body<NS, NF, RMAX, true>(o.slow[g - first], ins[b], rt->gps[b], o,
+            cell_ref_body<NS, NF, RMAX, true>(o.slow[g - first], ins[b], rt->gps + b, o,
                                               want_density, false, s_v, s_ns, s_cc, s_u, s_h,
                                               rt, b);
             __syncthreads();

@@ -197,9 +197,12 @@ void launch_cells_ref(const BlockIn &in, const GridP &gp, const CellOut &o,
                       int want_density, hipStream_t s);
 void launch_cells_ref_slow(const BlockIn &in, const GridP &gp, const CellOut &o,
                            int want_density, hipStream_t s);
-// several cells per wave in the geometry phase (cells_mc.hip); variant 1..4
-// picks cells per wave and capacities; returns -1 for an unknown variant
-int launch_cells_mc(const BlockIn &in, const GridP &gp, const CellOut &o, int want_density,
+// several cells per wave in the geometry phase (cells_mc.hip); variant 1..2
+// picks the verdict masks; returns -1 for an unknown variant.  gp is the
+// block's grid descriptor in device memory (read only by the scan, so that
+// its fields take no registers in the geometry phase; may be null without
+// density)
+int launch_cells_mc(const BlockIn &in, const GridP *gp, const CellOut &o, int want_density,
                     int variant, hipStream_t s);
 // large-capacity pass over the cells every block's fast pass deferred
 void launch_cells_ref_slow_batched(const BlockIn *ins, const Router *rt, int want_density,

@@ -356,6 +356,9 @@ def main():
     ap.add_argument("--calibrate", action="store_true",
                     help="run a plain copy kernel (256 MiB read, 256 MiB written) three times "
                          "and exit: the reference the HBM traffic counters are scaled by")
+    ap.add_argument("--cell-kernel", type=int, default=None,
+                    help="TESS2_OPT_CELL_KERNEL for the run (default: chosen per run; "
+                         "the autotuner keeps an explicit choice)")
     ap.add_argument("--no-tune", action="store_true",
                     help="several blocks: keep the default work split (no autotune), "
                          "for counter runs")
@@ -393,6 +396,8 @@ def main():
     grid = (gsz, gsz, gsz)
     t0 = time.time()
     run = host.StagedRun(blocks)
+    if args.cell_kernel is not None:
+        run.set_option("cell_kernel", args.cell_kernel)
     log(f"[stage] H2D upload {time.time() - t0:.2f}s")
     run.run(grid)
     run.sync()
@@ -448,16 +453,21 @@ def main():
     T = sum(b.num_tets for b in blocks)
     b_alg = 16 * N_all + 32 * T + 4 * N + 4 * G
     achieved = b_alg / (kern_ms * 1e-3) / 1e9
-    traffic = None
-    tf = os.path.join(ROOT, "profiles", f"traffic_{c['name']}.json")
-    if os.path.exists(tf):
-        traffic = json.load(open(tf)).get("hbm_bytes_per_launch")
     ck = run.get_option("cell_kernel")
     if multi:
         kernel = {1: "k_cells_mc_batched", 2: "k_cells_mc_batched",
                   4: "k_cells_mc_items"}.get(ck, "k_cells_ref_batched")
     else:
         kernel = "k_cells_ref" if ck == 0 else "k_cells_mc"
+    # HBM bytes per launch from the counter profile of this workload's
+    # dominant kernel, when there is one for the kernel this run used
+    traffic = None
+    tf = os.path.join(ROOT, "profiles", f"traffic_{c['name']}.json")
+    if os.path.exists(tf):
+        prof = json.load(open(tf))
+        for p in prof.get("kernels", [prof]):
+            if p.get("kernel") == kernel:
+                traffic = p.get("hbm_bytes_per_launch")
     cfg = {"workload": c["name"], "blocks": len(blocks), "particles": N, "ghosts": N_all - N,
            "tets": T, "grid": list(grid), "grid_points": G, "assignment": "reference_scan",
            "estimator": "DENSE_TESS", "steady_state": "hip_graph" if graph else "eager",

@@ -105,10 +105,19 @@ def test_graph_rerecords_on_new_arguments(oracle, single):
     assert np.array_equal(_bits(run.fetch(0)["density"]), _bits(ref["density"]))
 
 
-def test_graph_replay_exact(single):
+def test_graph_replay_exact(oracle, single):
+    """Replays of the exact assignment equal the eager run, and that run the
+    FP64 oracle: every owner, densities within the exact tests' 1e-5."""
     say("graph exact")
     kw = dict(glo_num_idx=(30, 30, 30), assignment=D.EXACT_VORONOI)
-    _replays([single], kw, exact=True)
+    run, want = _replays([single], kw, exact=True)
+    ref = oracle.exact_dense(oracle.Block.from_host(single), (30, 30, 30))
+    assert np.array_equal(want[0][1], ref["owner"])
+    den = want[0][0].view(np.float32).astype(np.float64)
+    w = ref["density"]
+    assert np.array_equal(den == 0, w == 0)
+    nz = w != 0
+    assert (np.abs(den[nz] - w[nz]) / np.abs(w[nz])).max() <= 1e-5
 
 
 def test_graph_replay_multiblock(oracle, multi):
@@ -143,7 +152,14 @@ def test_options_and_autotune_keep_results(oracle, multi):
         assert np.array_equal(_bits(run.fetch(i)["density"]), got[i]), i
 
 
-def test_graph_replay_exact_multiblock(multi):
+def test_graph_replay_exact_multiblock(oracle, multi):
+    """Replays over blocks equal the eager run, whose owners (home keys) are
+    the FP64 oracle's."""
+    from .test_oracle_exact_multi import home_keys
     say("graph exact multiblock")
     kw = dict(glo_num_idx=(33, 33, 33), assignment=D.EXACT_VORONOI)
-    _replays(multi, kw, exact=True)
+    run, want = _replays(multi, kw, exact=True)
+    ref = oracle.exact_dense_multi([oracle.Block.from_host(b) for b in multi], home_keys(multi),
+                                   (33, 33, 33))
+    for k in range(len(multi)):
+        assert np.array_equal(want[k][1], ref["owner"][k]), k

@@ -11,11 +11,14 @@ What pins each restated function:
   BlockGridParams (:575-648)     hand-derived indices; shared-face blocks
                                    partition the global index range.
   IterateCells / CellGridPts /   mass conservation; scan points are a subset of
-  CellInteriorGridPts             the brute-force PtInCell set; and the survey's
-    (:220-313,1363-1701)          measurements of the reference itself on a
-                                   re-creatable input (tests/golden/survey_p2.json):
-                                   deposited cells, scan misses, eps band and
-                                   exactly-matching cells all reproduce.
+  CellInteriorGridPts             the brute-force PtInCell set; and counts
+    (:220-313,1363-1701)          measured from the reference's scan built
+                                   against stand-in headers, on a re-creatable
+                                   input (tests/golden/reference_scan_counts.json,
+                                   corroboration): deposited cells, scan misses,
+                                   eps band and exactly-matching cells all
+                                   reproduce.  The scan's state machine itself is
+                                   pinned by test_oracle_known_answers.py.
   exact owner (FP64 oracle)      brute-force argmin with lowest-id ties.
 """
 import json
@@ -146,9 +149,10 @@ def _brute_ptincell(oracle, ob, c, gmin, step, eps):
     return set(map(tuple, pts.tolist())), lo, hi
 
 
-def test_survey_p2_aggregates(oracle):
-    """The reference's own scan on this input."""
-    gold = json.load(open(os.path.join(GOLDEN, "survey_p2.json")))["expected"]
+def test_reference_scan_counts(oracle):
+    """Counts measured from the reference's own scan (built against stand-in
+    MPI/DIY headers) on this input: corroboration of the oracle."""
+    gold = json.load(open(os.path.join(GOLDEN, "reference_scan_counts.json")))["expected"]
     hb, ob = _p2_block(oracle)
     r = oracle.dense_single(ob, (64, 64, 64))
     st = r["status"]

@@ -15,8 +15,8 @@ image: its sources include DIY headers that are absent):
   volume (src/volume.cpp:13-54)      periodic sum == 1; per cell == convex hull
                                      volume of its Voronoi vertices (FP64).
   fill_vert_to_tet (src/tess.cpp:767) last (highest) tet containing the vertex.
-BFS visit order itself is pinned by the survey's reference measurements
-(test_oracle_dense.py::test_survey_p2_aggregates).
+BFS visit order itself is pinned by hand-worked known answers
+(test_oracle_known_answers.py).
 """
 import numpy as np
 import pytest

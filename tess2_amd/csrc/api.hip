@@ -1087,14 +1087,16 @@ static int enqueue_kernels(tess2_ctx *c, const RunArgs &a)
     // counters of all blocks
     unsigned mcap = 0, mfold[3] = {0, 0, 0};
     int mcells = 0;
-    unsigned long long mgrid = 0;
+    unsigned long long mgrid = 0, sovf = 0, smid = 0;
     for (const DevBlock &d : c->blocks) {
         mcap = std::max(mcap, d.ovf_cap);
+        sovf += d.ovf_cap;
+        smid += d.fold_cap[1];
         for (int k = 0; k < 3; ++k) mfold[k] = std::max(mfold[k], d.fold_cap[k]);
         mcells = std::max(mcells, d.norig);
         mgrid = std::max<unsigned long long>(mgrid, d.ngrid);
     }
-    launch_fold(c->d_outs, c->d_gps, nb, mcap, mfold, mgrid, s);
+    launch_fold(c->d_outs, c->d_gps, nb, mcap, mfold, mgrid, sovf, smid, s);
     launch_stats(c->d_outs, c->d_gpart, nb, mcells, mgrid, s);
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipMemcpyAsync(c->h_stats, c->d_stats, sizeof(BlockStats) * nb,

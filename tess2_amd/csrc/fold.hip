@@ -76,10 +76,12 @@ __global__ __launch_bounds__(256) void k_fold_pairs(const CellOut *__restrict__ 
 }
 
 // 1. one segment per shared point; slot and segment reservations are
-// aggregated per workgroup (one atomic per workgroup and counter per pass)
-__global__ __launch_bounds__(256) void k_fold_gather(const CellOut *__restrict__ outs)
+// aggregated per workgroup (one atomic per workgroup and counter per pass of
+// 1024 entries: the four counters share a cache line, and a single-block run
+// with tens of millions of entries serialises on them)
+__global__ __launch_bounds__(1024) void k_fold_gather(const CellOut *__restrict__ outs)
 {
-    __shared__ unsigned s_cnt[4][4];   // per wave: points of class 0..2, segment total
+    __shared__ unsigned s_cnt[16][4];   // per wave: points of class 0..2, segment total
     const CellOut &o = outs[blockIdx.y];
     if (fold_overflowed(o)) return;
     const unsigned n = o.ctrs[C_OVF];
@@ -147,23 +149,96 @@ __global__ __launch_bounds__(256) void k_fold_gather(const CellOut *__restrict__
     }
 }
 
-// 2. list entries into their segments
-__global__ void k_fold_scatter(const CellOut *__restrict__ outs)
+// Passes 2 and 3b deal the items of all blocks (list entries, points) over
+// one 1-D grid: the blocks of a multi-block run differ in size by orders of
+// magnitude (a cluster core against its outskirts), and a grid per block left
+// the largest block's items to its own few workgroups.  Each workgroup builds
+// the exclusive prefix of the blocks' item counts in LDS (at most 128 blocks,
+// one lane per pair of blocks); an item finds its block by binary search.
+enum { FOLD_MAX_BLOCKS = 128 };
+
+// s_pre[b] = items of the blocks below b, s_pre[nb] = all items; items(b)
+// is read by the first wave
+template <typename F>
+__device__ __forceinline__ void fold_block_prefix(unsigned long long *s_pre, int nb, F items)
 {
-    const CellOut &o = outs[blockIdx.y];
-    if (fold_overflowed(o)) return;
-    const unsigned n = o.ctrs[C_OVF];
-    for (unsigned e = blockIdx.x * blockDim.x + threadIdx.x; e < n;
-         e += gridDim.x * blockDim.x) {
-        const int4 r = o.ovf[e];
+    if (threadIdx.x < 64) {
+        const int l = threadIdx.x, b0 = 2 * l, b1 = b0 + 1;
+        const unsigned long long c0 = b0 < nb ? items(b0) : 0ull;
+        const unsigned long long c1 = b1 < nb ? items(b1) : 0ull;
+        unsigned long long incl = c0 + c1;
+        for (int off = 1; off < 64; off <<= 1) {
+            const unsigned long long t = __shfl_up(incl, off);
+            if (l >= off) incl += t;
+        }
+        const unsigned long long ex = incl - c0 - c1;
+        if (b0 <= nb) s_pre[b0] = ex;
+        if (b1 <= nb) s_pre[b1] = ex + c0;
+        if (b1 + 1 == nb && l == 63) s_pre[nb] = incl;
+    }
+    __syncthreads();
+}
+
+// the block of item e < s_pre[nb]: the largest b with s_pre[b] <= e (an
+// empty block shares its start with the next one, so it is never chosen)
+__device__ __forceinline__ int fold_block_of(const unsigned long long *s_pre, int nb,
+                                             unsigned long long e)
+{
+    int lo = 0, hi = nb - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (s_pre[mid] <= e) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+// 2. list entries into their segments.  Entries of one point often sit side
+// by side in the list (neighbouring cells deposit to the same corner points
+// in the same step), so the lanes of a wave that fill one point's segment
+// take a single reservation: one atomic per point and wave, the lanes' slots
+// in lane order.  A wave can straddle two blocks, whose point records are
+// told apart by the block.  The fold sorts each segment, so slot order is
+// free.
+__global__ __launch_bounds__(256) void k_fold_scatter(const CellOut *__restrict__ outs, int nb)
+{
+    __shared__ unsigned long long s_pre[FOLD_MAX_BLOCKS + 1];
+    fold_block_prefix(s_pre, nb, [&](int b) -> unsigned long long {
+        const CellOut &o = outs[b];
+        return fold_overflowed(o) ? 0ull : (unsigned long long)o.ctrs[C_OVF];
+    });
+    const int lane = threadIdx.x & 63;
+    const unsigned long long below = (1ull << lane) - 1ull;
+    const unsigned long long n = s_pre[nb];
+    for (unsigned long long e = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x; e < n;
+         e += (unsigned long long)gridDim.x * blockDim.x) {
+        const int b = fold_block_of(s_pre, nb, e);
+        const CellOut &o = outs[b];
+        const int4 r = o.ovf[(unsigned)(e - s_pre[b])];
         const int h = o.head[r.x];
-        if (h >= 0) continue;
         const int enc = -h - 1;
-        const int cls = enc & 3;
-        if (cls == 3) continue;
-        int4 *rec = fold_list(o, cls) + (enc >> 2);
-        const unsigned k = atomicAdd(reinterpret_cast<unsigned *>(&rec->w), 1u);
-        o.seg[(unsigned)rec->y + k] = make_uint2((unsigned)r.y, (unsigned)r.z);
+        const bool ok = h < 0 && (enc & 3) != 3;
+        int lead = lane;
+        unsigned rank = 0, gn = 0;
+        unsigned long long pend = __ballot(ok);
+        while (pend) {   // one round per distinct (block, point) among the lanes
+            const int l = __ffsll((long long)pend) - 1;
+            const int key = __builtin_amdgcn_readlane(enc, l);
+            const int kb = __builtin_amdgcn_readlane(b, l);
+            const bool in = ok && enc == key && b == kb;
+            const unsigned long long m = __ballot(in);
+            if (in) {
+                lead = l;
+                rank = (unsigned)__popcll(m & below);
+                gn = (unsigned)__popcll(m);
+            }
+            pend &= ~m;
+        }
+        int4 *rec = fold_list(o, enc & 3) + (enc >> 2);
+        unsigned base = 0;
+        if (ok && lane == lead) base = atomicAdd(reinterpret_cast<unsigned *>(&rec->w), gn);
+        base = __shfl(base, lead);
+        if (ok) o.seg[(unsigned)rec->y + base + rank] = make_uint2((unsigned)r.y, (unsigned)r.z);
     }
 }
 
@@ -292,17 +367,22 @@ __device__ __forceinline__ int pow2_at_least(int x)
 
 // 3b. 17..2048 contributions: one wave per point, bitonic sort in LDS
 __global__ __launch_bounds__(64) void k_fold_mid(const CellOut *__restrict__ outs,
-                                                 const GridP *__restrict__ gps)
+                                                 const GridP *__restrict__ gps, int nb)
 {
     __shared__ unsigned sk[FOLD_MID_MAX], sm[FOLD_MID_MAX];
-    const CellOut &o = outs[blockIdx.y];
-    const float div = gps[blockIdx.y].div;
-    if (fold_overflowed(o)) return;
-    unsigned n = o.ctrs[C_FOLD0 + 1];
-    if (n > o.fold_cap[1]) n = o.fold_cap[1];
-    const double dd = (double)div;
-    for (unsigned i = blockIdx.x; i < n; i += gridDim.x) {
-        const int4 r = o.fold[1][i];
+    __shared__ unsigned long long s_pre[FOLD_MAX_BLOCKS + 1];
+    fold_block_prefix(s_pre, nb, [&](int b) -> unsigned long long {
+        const CellOut &o = outs[b];
+        if (fold_overflowed(o)) return 0ull;
+        const unsigned n = o.ctrs[C_FOLD0 + 1];
+        return n < o.fold_cap[1] ? n : o.fold_cap[1];
+    });
+    const unsigned long long n = s_pre[nb];
+    for (unsigned long long e = blockIdx.x; e < n; e += gridDim.x) {
+        const int b = __builtin_amdgcn_readfirstlane(fold_block_of(s_pre, nb, e));
+        const CellOut &o = outs[b];
+        const double dd = (double)gps[b].div;
+        const int4 r = o.fold[1][(unsigned)(e - s_pre[b])];
         const uint2 *sg = o.seg + (unsigned)r.y;
         const int cnt = r.z;
         const int P = pow2_at_least(cnt);
@@ -413,21 +493,24 @@ static unsigned capped_blocks(unsigned long long work, unsigned per, unsigned ca
 }
 
 void launch_fold(const CellOut *outs, const GridP *gps, int nb, unsigned max_ovf_cap,
-                 const unsigned max_fold_cap[3], unsigned long long max_grid, hipStream_t s)
+                 const unsigned max_fold_cap[3], unsigned long long max_grid,
+                 unsigned long long sum_ovf_cap, unsigned long long sum_mid_cap, hipStream_t s)
 {
     if (nb <= 0) return;
-    // grids sized by the largest block's capacities, fewer workgroups per
-    // block when there are many blocks (the loops stride)
+    // per-block grids sized by the largest block's capacities, fewer
+    // workgroups per block when there are many blocks (the loops stride);
+    // the dealt passes are sized by the capacities of all blocks
     const unsigned per = nb > 16 ? 256 : 4096;
-    const unsigned b = capped_blocks(max_ovf_cap, 256, per);
     const unsigned bg = (unsigned)std::min<unsigned long long>(per, max_grid / 256 + 1);
     hipLaunchKernelGGL(k_fold_pairs, dim3(bg, nb), dim3(256), 0, s, outs, gps);
-    hipLaunchKernelGGL(k_fold_gather, dim3(b, nb), dim3(256), 0, s, outs);
-    hipLaunchKernelGGL(k_fold_scatter, dim3(b, nb), dim3(256), 0, s, outs);
+    hipLaunchKernelGGL(k_fold_gather, dim3(capped_blocks(max_ovf_cap, 1024, per / 4), nb),
+                       dim3(1024), 0, s, outs);
+    hipLaunchKernelGGL(k_fold_scatter, dim3(capped_blocks(sum_ovf_cap, 256, 4096)), dim3(256), 0,
+                       s, outs, nb);
     hipLaunchKernelGGL(k_fold_small, dim3(capped_blocks(max_fold_cap[0], 256, per), nb),
                        dim3(256), 0, s, outs, gps);
-    hipLaunchKernelGGL(k_fold_mid, dim3(capped_blocks(max_fold_cap[1], 1, per), nb), dim3(64), 0,
-                       s, outs, gps);
+    hipLaunchKernelGGL(k_fold_mid, dim3(capped_blocks(sum_mid_cap, 1, 8192)), dim3(64), 0, s,
+                       outs, gps, nb);
     hipLaunchKernelGGL(k_fold_large, dim3(capped_blocks(max_fold_cap[2], 1, 64), nb),
                        dim3(1024), 0, s, outs, gps);
 }

@@ -227,8 +227,10 @@ void launch_cells_ref_batched(const BlockIn *ins, const GridP *gps, const CellOu
                               bool spec);
 // ordered fold of shared grid points and per-block statistics for nb blocks
 // at once (outs/gps: device arrays; the capacities size the grids)
+// nb <= 128; the sums are the list capacities of all blocks
 void launch_fold(const CellOut *outs, const GridP *gps, int nb, unsigned max_ovf_cap,
-                 const unsigned max_fold_cap[3], unsigned long long max_grid, hipStream_t s);
+                 const unsigned max_fold_cap[3], unsigned long long max_grid,
+                 unsigned long long sum_ovf_cap, unsigned long long sum_mid_cap, hipStream_t s);
 // part: nb * 1024 doubles of scratch for the fixed-order grid sums
 void launch_stats(const CellOut *outs, double *part, int nb, int max_cells,
                   unsigned long long max_grid, hipStream_t s);

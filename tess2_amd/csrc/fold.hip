@@ -265,6 +265,22 @@ template <int P> __device__ __forceinline__ void oe_sort(unsigned *kk, unsigned 
     }
 }
 
+// the chain of the first cnt sorted register entries, unrolled over the
+// network's size (static register indices, selects rather than branches;
+// entries past cnt are skipped, not added as zeros, which would turn a -0
+// sum into +0)
+template <int P>
+__device__ __forceinline__ float reg_chain(const unsigned *mm, int cnt, double dd)
+{
+    float rho = 0.0f;
+#pragma unroll
+    for (int j = 0; j < P; ++j) {
+        const float t = (float)((double)rho + (double)__uint_as_float(mm[j]) / dd);
+        rho = j < cnt ? t : rho;
+    }
+    return rho;
+}
+
 // 3a. at most 16 contributions: one lane per point, odd-even transposition
 // sort in registers
 __global__ void k_fold_small(const CellOut *__restrict__ outs, const GridP *__restrict__ gps)
@@ -278,7 +294,7 @@ __global__ void k_fold_small(const CellOut *__restrict__ outs, const GridP *__re
     for (unsigned i = blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += gridDim.x * blockDim.x) {
         const int4 r = o.fold[0][i];
-        uint2 *sg = o.seg + (unsigned)r.y;
+        const uint2 *sg = o.seg + (unsigned)r.y;
         const int cnt = r.z;
         unsigned kk[FOLD_SMALL_MAX], mm[FOLD_SMALL_MAX];
 #pragma unroll
@@ -291,21 +307,23 @@ __global__ void k_fold_small(const CellOut *__restrict__ outs, const GridP *__re
                 mm[j] = q.y;
             }
         }
-        // the network spans the wave's largest count rounded up to 2, 4, 8
-        // or 16 (most shared points have two contributions); the unused
-        // entries hold the largest key and stay at the end
-        if (__ballot(cnt > 8)) oe_sort<16>(kk, mm);
-        else if (__ballot(cnt > 4)) oe_sort<8>(kk, mm);
-        else if (__ballot(cnt > 2)) oe_sort<4>(kk, mm);
-        else oe_sort<2>(kk, mm);
-        // sorted back into the segment, then one rolled chain (no dynamic
-        // register indexing)
-#pragma unroll
-        for (int j = 0; j < FOLD_SMALL_MAX; ++j)
-            if (j < cnt) sg[j] = make_uint2(kk[j], mm[j]);
-        float rho = 0.0f;
-        for (int j = 0; j < cnt; ++j)
-            rho = (float)((double)rho + (double)__uint_as_float(sg[j].y) / dd);
+        // the network and the chain span the wave's largest count rounded
+        // up to 2, 4, 8 or 16 (most shared points have two contributions);
+        // the unused entries hold the largest key and stay at the end
+        float rho;
+        if (__ballot(cnt > 8)) {
+            oe_sort<16>(kk, mm);
+            rho = reg_chain<16>(mm, cnt, dd);
+        } else if (__ballot(cnt > 4)) {
+            oe_sort<8>(kk, mm);
+            rho = reg_chain<8>(mm, cnt, dd);
+        } else if (__ballot(cnt > 2)) {
+            oe_sort<4>(kk, mm);
+            rho = reg_chain<4>(mm, cnt, dd);
+        } else {
+            oe_sort<2>(kk, mm);
+            rho = reg_chain<2>(mm, cnt, dd);
+        }
         o.density[r.x] = rho;
     }
 }

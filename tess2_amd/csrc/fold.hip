@@ -193,13 +193,9 @@ __device__ __forceinline__ int fold_block_of(const unsigned long long *s_pre, in
     return lo;
 }
 
-// 2. list entries into their segments.  Entries of one point often sit side
-// by side in the list (neighbouring cells deposit to the same corner points
-// in the same step), so the lanes of a wave that fill one point's segment
-// take a single reservation: one atomic per point and wave, the lanes' slots
-// in lane order.  A wave can straddle two blocks, whose point records are
-// told apart by the block.  The fold sorts each segment, so slot order is
-// free.
+// 2. list entries into their segments (one slot reservation per entry: a
+// point's entries seldom share a wave, and grouping the lanes of a wave by
+// point measured no faster)
 __global__ __launch_bounds__(256) void k_fold_scatter(const CellOut *__restrict__ outs, int nb)
 {
     __shared__ unsigned long long s_pre[FOLD_MAX_BLOCKS + 1];
@@ -207,8 +203,6 @@ __global__ __launch_bounds__(256) void k_fold_scatter(const CellOut *__restrict_
         const CellOut &o = outs[b];
         return fold_overflowed(o) ? 0ull : (unsigned long long)o.ctrs[C_OVF];
     });
-    const int lane = threadIdx.x & 63;
-    const unsigned long long below = (1ull << lane) - 1ull;
     const unsigned long long n = s_pre[nb];
     for (unsigned long long e = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x; e < n;
          e += (unsigned long long)gridDim.x * blockDim.x) {
@@ -216,29 +210,13 @@ __global__ __launch_bounds__(256) void k_fold_scatter(const CellOut *__restrict_
         const CellOut &o = outs[b];
         const int4 r = o.ovf[(unsigned)(e - s_pre[b])];
         const int h = o.head[r.x];
+        if (h >= 0) continue;
         const int enc = -h - 1;
-        const bool ok = h < 0 && (enc & 3) != 3;
-        int lead = lane;
-        unsigned rank = 0, gn = 0;
-        unsigned long long pend = __ballot(ok);
-        while (pend) {   // one round per distinct (block, point) among the lanes
-            const int l = __ffsll((long long)pend) - 1;
-            const int key = __builtin_amdgcn_readlane(enc, l);
-            const int kb = __builtin_amdgcn_readlane(b, l);
-            const bool in = ok && enc == key && b == kb;
-            const unsigned long long m = __ballot(in);
-            if (in) {
-                lead = l;
-                rank = (unsigned)__popcll(m & below);
-                gn = (unsigned)__popcll(m);
-            }
-            pend &= ~m;
-        }
-        int4 *rec = fold_list(o, enc & 3) + (enc >> 2);
-        unsigned base = 0;
-        if (ok && lane == lead) base = atomicAdd(reinterpret_cast<unsigned *>(&rec->w), gn);
-        base = __shfl(base, lead);
-        if (ok) o.seg[(unsigned)rec->y + base + rank] = make_uint2((unsigned)r.y, (unsigned)r.z);
+        const int cls = enc & 3;
+        if (cls == 3) continue;
+        int4 *rec = fold_list(o, cls) + (enc >> 2);
+        const unsigned k = atomicAdd(reinterpret_cast<unsigned *>(&rec->w), 1u);
+        o.seg[(unsigned)rec->y + k] = make_uint2((unsigned)r.y, (unsigned)r.z);
     }
 }
 

@@ -97,7 +97,8 @@ enum {
     TESS2_E_HIP = -2,
     TESS2_E_CAPACITY = -3,   /* a per-cell capacity was exceeded */
     TESS2_E_NOMEM = -4,
-    TESS2_E_IO = -5
+    TESS2_E_IO = -5,
+    TESS2_E_INTERNAL = -6    /* an internal consistency check failed: outputs are invalid */
 };
 
 /* counters of one dense run (the reference prints tot_mass / check_mass /

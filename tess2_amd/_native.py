@@ -18,7 +18,8 @@ ASSIGN_REFERENCE_SCAN, ASSIGN_EXACT_VORONOI = 0, 1
 PREC_MIXED, PREC_FP64 = 0, 1
 
 ERRORS = {0: "ok", -1: "invalid argument", -2: "HIP error", -3: "capacity exceeded",
-          -4: "out of device memory", -5: "I/O error"}
+          -4: "out of device memory", -5: "I/O error",
+          -6: "internal consistency check failed"}
 
 
 class Tess2Error(RuntimeError):

@@ -1222,6 +1222,9 @@ extern "C" int tess2_sync(tess2_ctx *c, tess2_stats_t *st)
             if (d.hm->ctrs[C_ERR] & 8u)
                 return fail(c, TESS2_E_CAPACITY, "block %d: shared grid point beyond the fold's "
                             "capacities", d.gid);
+            if (d.hm->ctrs[C_ERR] & 32u)
+                return fail(c, TESS2_E_INTERNAL, "block %d: a shared-list entry's ordinal lies "
+                            "outside its point's segment", d.gid);
         }
         if (!retry) break;
         if (attempt >= 4) return fail(c, TESS2_E_CAPACITY, "list capacities did not converge");

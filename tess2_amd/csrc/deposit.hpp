@@ -41,9 +41,11 @@ __device__ __forceinline__ void push_point(long long lidx, float m, unsigned key
         if (e < o.ovf_cap) {
             // head only has to name one list entry of the point (the fold
             // takes that entry as the point's representative): a plain store,
-            // the last writer wins, no round trip
+            // the last writer wins, no round trip.  The entry keeps its
+            // ordinal c among the point's contributions: the fold's scatter
+            // puts it at slot c - 2 of the point's segment, with no atomic
             o.head[lidx] = (int)e;
-            o.ovf[e] = make_int4((int)lidx, (int)key, __float_as_int(m), 0);
+            o.ovf[e] = make_int4((int)lidx, (int)key, __float_as_int(m), (int)c);
         }
     }
 }

@@ -193,9 +193,11 @@ __device__ __forceinline__ int fold_block_of(const unsigned long long *s_pre, in
     return lo;
 }
 
-// 2. list entries into their segments (one slot reservation per entry: a
-// point's entries seldom share a wave, and grouping the lanes of a wave by
-// point measured no faster)
+// 2. list entries into their segments.  Each entry carries its ordinal c
+// among its point's contributions (the count the deposit's atomic returned),
+// and the first two sit at the segment's end, so the entry's slot is c - 2:
+// no atomic, no contention on points with many contributions.  An ordinal
+// outside the segment (an internal error) is flagged, not written.
 __global__ __launch_bounds__(256) void k_fold_scatter(const CellOut *__restrict__ outs, int nb)
 {
     __shared__ unsigned long long s_pre[FOLD_MAX_BLOCKS + 1];
@@ -214,9 +216,13 @@ __global__ __launch_bounds__(256) void k_fold_scatter(const CellOut *__restrict_
         const int enc = -h - 1;
         const int cls = enc & 3;
         if (cls == 3) continue;
-        int4 *rec = fold_list(o, cls) + (enc >> 2);
-        const unsigned k = atomicAdd(reinterpret_cast<unsigned *>(&rec->w), 1u);
-        o.seg[(unsigned)rec->y + k] = make_uint2((unsigned)r.y, (unsigned)r.z);
+        const int4 q = fold_list(o, cls)[enc >> 2];   // {point, start, count, 0}
+        const unsigned k = (unsigned)r.w - 2u;
+        if (k >= (unsigned)q.z - 2u) {
+            atomicOr(&o.ctrs[C_ERR], 32u);
+            continue;
+        }
+        o.seg[(unsigned)q.y + k] = make_uint2((unsigned)r.y, (unsigned)r.z);
     }
 }
 

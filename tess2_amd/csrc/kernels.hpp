@@ -65,7 +65,8 @@ struct CellOut {
     uint2 *first;      // [grid] first contribution (cell, mass bits)
     uint2 *second;     // [grid] second contribution (the list holds the third on)
     int *head;         // [grid] one shared-list entry of the point (its representative)
-    int4 *ovf;         // shared-point list, third contribution on: {lidx, key, mass bits, 0};
+    int4 *ovf;         // shared-point list, third contribution on: {lidx, key, mass bits,
+                       // ordinal among the point's contributions (2 for the third)};
                        // (3 * ovf_cap + 1) / 2 entries, the large fold's scratch
     unsigned ovf_cap;
     unsigned *ctrs;    // [C_NCTR]

@@ -387,16 +387,45 @@ __global__ __launch_bounds__(64) void k_fold_mid(const CellOut *__restrict__ out
         const int4 r = o.fold[1][(unsigned)(e - s_pre[b])];
         const uint2 *sg = o.seg + (unsigned)r.y;
         const int cnt = r.z;
-        const int P = pow2_at_least(cnt);
-        for (int t = threadIdx.x; t < P; t += 64) {
-            uint2 q = make_uint2(0xffffffffu, 0u);
+        if (cnt <= 64) {
+            // one entry per lane: bitonic network over the wave by lane
+            // exchanges, no LDS round trips or barriers; the quotients go
+            // to LDS in sorted order for the chain
+            const int t = threadIdx.x;
+            uint2 q = make_uint2(0xffffffffu, 0u);   // padding sorts last
             if (t < cnt) q = sg[t];
-            sk[t] = q.x;
-            sm[t] = q.y;
+            unsigned k = q.x, m = q.y;
+#pragma unroll
+            for (int sz = 2; sz <= 64; sz <<= 1) {
+#pragma unroll
+                for (int j = sz >> 1; j > 0; j >>= 1) {
+                    const unsigned pk = __shfl_xor(k, j), pm = __shfl_xor(m, j);
+                    // the lower lane of a pair keeps the smaller key in an
+                    // ascending run, the larger in a descending one
+                    const bool keep_min = ((t & j) == 0) == ((t & sz) == 0);
+                    if (keep_min ? pk < k : pk > k) {
+                        k = pk;
+                        m = pm;
+                    }
+                }
+            }
+            const double qd = (double)__uint_as_float(m) / dd;
+            const unsigned long long b = (unsigned long long)__double_as_longlong(qd);
+            sk[t] = (unsigned)b;
+            sm[t] = (unsigned)(b >> 32);
+            __syncthreads();
+        } else {
+            const int P = pow2_at_least(cnt);
+            for (int t = threadIdx.x; t < P; t += 64) {
+                uint2 q = make_uint2(0xffffffffu, 0u);
+                if (t < cnt) q = sg[t];
+                sk[t] = q.x;
+                sm[t] = q.y;
+            }
+            __syncthreads();
+            lds_bitonic(sk, sm, P);
+            lds_quotients(sk, sm, cnt, dd);
         }
-        __syncthreads();
-        lds_bitonic(sk, sm, P);
-        lds_quotients(sk, sm, cnt, dd);
         if (threadIdx.x == 0) o.density[r.x] = lds_chain(sk, sm, cnt, 0.0f);
         __syncthreads();
     }

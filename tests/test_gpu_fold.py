@@ -70,3 +70,20 @@ def test_heavy_shared_points_cic(oracle, blob):
     run.sync()
     want = oracle.dense_single(oracle.Block.from_host(blob), (17, 17, 17), cic=True)["density"]
     assert np.array_equal(_bits(run.fetch(0)["density"]), _bits(want))
+
+
+@pytest.mark.parametrize("n_blob", [40, 63, 100, 600])
+def test_mid_shared_points_bitexact(oracle, n_blob):
+    """Smaller blobs put a few grid points in the one-wave class (17-2048
+    contributions): about 40 and 63 land in the register sort (at most 64),
+    100 and 600 in the LDS sort; both estimators."""
+    blk = _blob(n_blob, 2000, 0.002, 72)
+    g = (17, 17, 17)
+    for alg, cic in ((D.DENSE_TESS, False), (D.DENSE_CIC, True)):
+        say("fold mid blob", n_blob, "cic" if cic else "tess")
+        run = D.StagedRun([blk])
+        run.run(g, alg=alg)
+        st = run.sync()
+        assert st["points_shared"] >= n_blob - 2
+        want = oracle.dense_single(oracle.Block.from_host(blk), g, cic=cic)["density"]
+        assert np.array_equal(_bits(run.fetch(0)["density"]), _bits(want))

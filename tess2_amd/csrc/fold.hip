@@ -13,11 +13,13 @@
 //   1. gather: each larger point (found through its list head) reserves a
 //      contiguous segment, sized by its contribution count, and is filed in
 //      one of three classes by that count;
-//   2. scatter: list entries go to their point's segment;
+//   2. scatter: list entries go to their point's segment, each at the slot
+//      its ordinal among the point's contributions gives;
 //   3. fold: sort each segment by key and add in order -- in registers (at
-//      most 16 contributions, one lane per point), in LDS (at most 2048, one
-//      wave per point), or by chunk sorts in LDS plus merge passes in global
-//      memory (larger, one 1024-lane workgroup per point).
+//      most 16 contributions, one lane per point), in one wave per point (at
+//      most 2048: by lane exchanges up to 64, else in LDS), or by chunk sorts
+//      in LDS plus merge passes in global memory (larger, one 1024-lane
+//      workgroup per point).
 // The additions of one point are a chain of dependent float roundings, so
 // each runs on a single lane; only the division by div is done in parallel.
 #include <algorithm>

@@ -38,15 +38,10 @@ __device__ __forceinline__ void push_point(long long lidx, float m, unsigned key
         o.second[lidx] = make_uint2(key, __float_as_uint(m));
     } else {
         const unsigned e = atomicAdd(&o.ctrs[C_OVF], 1u);
-        if (e < o.ovf_cap) {
-            // head only has to name one list entry of the point (the fold
-            // takes that entry as the point's representative): a plain store,
-            // the last writer wins, no round trip.  The entry keeps its
-            // ordinal c among the point's contributions: the fold's scatter
-            // puts it at slot c - 2 of the point's segment, with no atomic
-            o.head[lidx] = (int)e;
-            o.ovf[e] = make_int4((int)lidx, (int)key, __float_as_int(m), (int)c);
-        }
+        // the entry keeps its ordinal c among the point's contributions: the
+        // entry with c == 2 represents the point in the fold, and the fold's
+        // scatter puts each entry at slot c - 2 of the point's segment
+        if (e < o.ovf_cap) o.ovf[e] = make_int4((int)lidx, (int)key, __float_as_int(m), (int)c);
     }
 }
 

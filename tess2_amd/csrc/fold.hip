@@ -10,9 +10,10 @@
 // than one in ascending key order (key = cell id, or the received-point key
 // above 2^31):
 //   0. pairs: a grid pass adds the points with exactly two contributions;
-//   1. gather: each larger point (found through its list head) reserves a
-//      contiguous segment, sized by its contribution count, and is filed in
-//      one of three classes by that count;
+//   1. gather: each larger point (through its representative list entry,
+//      the third contribution) reserves a contiguous segment, sized by its
+//      contribution count, is filed in one of three classes by that count,
+//      and links its record in head;
 //   2. scatter: list entries go to their point's segment, each at the slot
 //      its ordinal among the point's contributions gives;
 //   3. fold: sort each segment by key and add in order -- in registers (at
@@ -97,7 +98,7 @@ __global__ __launch_bounds__(1024) void k_fold_gather(const CellOut *__restrict_
         if (e < n) {
             const int4 r = o.ovf[e];
             lidx = r.x;
-            mine = o.head[lidx] == (int)e;   // the entry that represents its point
+            mine = r.w == 2;   // the point's third contribution represents it
             if (mine) {
                 cnt = o.count[lidx];
                 cls = cnt <= FOLD_SMALL_MAX ? 0 : (cnt <= FOLD_MID_MAX ? 1 : 2);

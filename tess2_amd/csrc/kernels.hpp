@@ -64,7 +64,8 @@ struct CellOut {
     unsigned *count;   // [grid] contributions per point
     uint2 *first;      // [grid] first contribution (cell, mass bits)
     uint2 *second;     // [grid] second contribution (the list holds the third on)
-    int *head;         // [grid] one shared-list entry of the point (its representative)
+    int *head;         // [grid] the point's fold record, -(1 + 4 index + class), written
+                       // by the fold's gather for points with list entries
     int4 *ovf;         // shared-point list, third contribution on: {lidx, key, mass bits,
                        // ordinal among the point's contributions (2 for the third)};
                        // (3 * ovf_cap + 1) / 2 entries, the large fold's scratch

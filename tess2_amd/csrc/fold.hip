@@ -370,7 +370,8 @@ __device__ __forceinline__ int pow2_at_least(int x)
     return p;
 }
 
-// 3b. 17..2048 contributions: one wave per point, bitonic sort in LDS
+// 3b. 17..2048 contributions: one wave per point, bitonic sort by lane
+// exchanges (at most 64) or in LDS
 __global__ __launch_bounds__(64) void k_fold_mid(const CellOut *__restrict__ outs,
                                                  const GridP *__restrict__ gps, int nb)
 {
@@ -413,9 +414,9 @@ __global__ __launch_bounds__(64) void k_fold_mid(const CellOut *__restrict__ out
                 }
             }
             const double qd = (double)__uint_as_float(m) / dd;
-            const unsigned long long b = (unsigned long long)__double_as_longlong(qd);
-            sk[t] = (unsigned)b;
-            sm[t] = (unsigned)(b >> 32);
+            const unsigned long long qb = (unsigned long long)__double_as_longlong(qd);
+            sk[t] = (unsigned)qb;
+            sm[t] = (unsigned)(qb >> 32);
             __syncthreads();
         } else {
             const int P = pow2_at_least(cnt);
